@@ -1,0 +1,158 @@
+/*
+ * scadyn_amd.h -- C ABI of the MI355X JVIE T-matrix solve.
+ *
+ * The reference (scadyn, Fortran) has no FFI on this path: calc_T
+ * (src/T_matrix.f90:18-82) calls the module procedures update_projections,
+ * build_G, compute_near_zone_interactions_const and compute_T_matrix
+ * (src/T_matrix.f90:56-72) on the global `mesh`/`matrices` state.  This
+ * header is the boundary that replaces those four calls with one call per
+ * wavelength bar, bound from Fortran with ISO_C_BINDING
+ * (scadyn_amd/fortran/scadyn_amd_iface.f90).
+ *
+ * Conventions (identical to the reference's in-memory layouts):
+ *   coord   double[3*n_node]   Fortran coord(3, n_node), metres, already
+ *                              rescaled to the volume-equivalent radius
+ *                              (src/h5io.f90:128-131)
+ *   etopol  int32 [4*n_tet]    Fortran etopol(4, n_tet), 1-based node ids
+ *   param   double[2*n_tet]    complex(dp) param(n_tet): relative permittivity
+ *                              of each tet for this bar (src/T_matrix.f90:37-43)
+ *   complex vectors/matrices   interleaved (re, im) doubles, i.e. Fortran
+ *                              complex(c_double_complex); matrices column-major
+ *   unknown vector x(3N)       3*(t-1)+c: component c of tet t
+ *                              (src/gmres_module.f90:80-86)
+ *
+ * Every call is synchronous, copies its inputs to the device and its
+ * results back, and keeps no host pointer after it returns.  All calls
+ * return 0 on success or a SCADYN_ERR_* code; scadyn_last_error() gives
+ * the message of the last failure in the calling thread.
+ */
+#ifndef SCADYN_AMD_H
+#define SCADYN_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SCADYN_OK               0
+#define SCADYN_ERR_ARG          1  /* bad argument or size                          */
+#define SCADYN_ERR_ORIENTATION  2  /* detJ <= 0 (src/common.f90:466-469)           */
+#define SCADYN_ERR_GRID         3  /* centroid outside the cube grid (geometry.f90) */
+#define SCADYN_ERR_SINGULAR     4  /* singular least-squares / triangular system    */
+#define SCADYN_ERR_NOCONV       5  /* GMRES did not reach tol (strict mode only)     */
+#define SCADYN_ERR_MEMORY       6  /* device memory budget exceeded                  */
+#define SCADYN_ERR_DEVICE       7  /* HIP runtime failure                            */
+#define SCADYN_ERR_UNSUPPORTED  8  /* option the build does not implement            */
+
+#define SCADYN_PREC_FP64   0  /* FP64 matvec, reference-faithful GMRES          */
+#define SCADYN_PREC_MIXED  1  /* FP32 matvec inside FP64 iterative refinement   */
+
+typedef struct scadyn_mesh {
+    int32_t n_node;
+    int32_t n_tet;
+    const double *coord;
+    const int32_t *etopol;
+    const double *param;
+} scadyn_mesh;
+
+/* Solver options; field meanings follow mesh_struct (src/common.f90:78-89). */
+typedef struct scadyn_opts {
+    double tol;          /* GMRES tolerance on |g(i+1)|/||b||   (mesh%tol, 1e-5)       */
+    int32_t maxit;       /* Krylov dimension per cycle            (mesh%maxit, 50)      */
+    int32_t restart;     /* number of restart cycles              (mesh%restart, 4)     */
+    int32_t M_ex;        /* stencil order, only 2                 (mesh%M_ex)           */
+    int32_t near_zone;   /* near-zone radius in cubes, only 1     (mesh%near_zone)      */
+    double grid_size;    /* grid step factor                      (mesh%grid_size, 1.8) */
+    int32_t order;       /* basis order, only 0                   (mesh%order)          */
+    int32_t precision;   /* SCADYN_PREC_*                                               */
+    int32_t batch;       /* RHS slots solved concurrently, 0 = automatic                */
+    int32_t strict;      /* nonzero: non-convergence returns SCADYN_ERR_NOCONV          */
+    int32_t device;      /* HIP device ordinal                                          */
+    int32_t fft_pad;     /* 0: smallest 2,3,5,7-smooth length >= 2N-1 per axis;
+                            1: the reference's 2N                                       */
+    double refine_tol;   /* mixed precision: outer FP64 relative residual target        */
+    int32_t reserved[8];
+} scadyn_opts;
+
+/* Defaults of the reference (src/common.f90:78-89). */
+void scadyn_opts_default(scadyn_opts *opts);
+
+/* Message of the last failure in this thread ("" if none). */
+const char *scadyn_last_error(void);
+
+/*
+ * One wavelength bar: replaces update_projections + build_G +
+ * compute_near_zone_interactions_const + compute_T_matrix
+ * (src/T_matrix.f90:56-72).  k is the wavenumber (1/m), nmax the expansion
+ * order; Taa, Tab, Tba, Tbb are caller-allocated nm x nm column-major
+ * complex arrays, nm = nmax(nmax+2), returned already scaled by i k^3
+ * (src/T_matrix.f90:270-275).  iters (optional, length 2*nm) receives
+ * the GMRES iteration count of each column.
+ */
+int scadyn_tmat_solve(const scadyn_mesh *mesh, const scadyn_opts *opts,
+                      double k, int32_t nmax,
+                      double *Taa, double *Tab, double *Tba, double *Tbb,
+                      int32_t *iters);
+
+/* ---- operator-level API (the request loop, tests, benchmarks) -------- */
+
+typedef struct scadyn_operator scadyn_operator;
+
+/* Device-resident pFFT operator of one (mesh, k): grid, projections S*,
+ * FFT(G), near-zone blocks (src/setup.f90:77-140, src/build_G.f90,
+ * src/precorrection.f90:131-264). */
+int scadyn_op_create(const scadyn_mesh *mesh, const scadyn_opts *opts, double k,
+                     scadyn_operator **out);
+void scadyn_op_destroy(scadyn_operator *op);
+
+/* Sizes: info[0..15] = N_tet, Nx, Ny, Nz, N_cubes, N_tet_cube, n_blocks,
+ * Lx, Ly, Lz, batch, device bytes, 0... ; delta and box_delta in dinfo[0..1]. */
+int scadyn_op_info(const scadyn_operator *op, int64_t info[16], double dinfo[4]);
+
+#define SCADYN_EXPORT_S       1  /* S,Sx,Sy,Sz: complex [4][N][27], reference tet order  */
+#define SCADYN_EXPORT_BLOCKS  2  /* near-zone blocks: complex64 [n_blocks][9] col-major,
+                                    reference order (src/precorrection.f90:182-258)     */
+#define SCADYN_EXPORT_BLKIND  3  /* int32 [n_blocks][2] 1-based (tet_T, tet_B)           */
+#define SCADYN_EXPORT_FG      4  /* FFT(G) octant, complex [Lz/2+1][Ly/2+1][Lx/2+1],
+                                    scaled by 1/(Lx Ly Lz)                             */
+/* Copy an internal array to host memory (nbytes must match exactly). */
+int scadyn_op_export(const scadyn_operator *op, int32_t what, void *host, int64_t nbytes);
+
+/* y = A x for nrhs vectors x[nrhs][3N] (compute_Ax2, src/gmres_module.f90:57-176). */
+int scadyn_op_matvec(scadyn_operator *op, int32_t nrhs, const double *x, double *y);
+
+/* Solve A x = rhs for nrhs right-hand sides [nrhs][3N] with restarted GMRES
+ * (src/gmres_module.f90:180-325).  iters[nrhs] (optional) receives the
+ * reference's iteration count `ite` of each system. */
+int scadyn_op_solve(scadyn_operator *op, int32_t nrhs, const double *rhs, double *x,
+                    int32_t *iters);
+
+/* Regular-VSWF projection matrix mat(3N, 2nm) column-major, reference tet order
+ * (vswf2constant, src/T_matrix.f90:280-305). */
+int scadyn_op_vswf(scadyn_operator *op, int32_t nmax, double *mat);
+
+/* T-matrix blocks of this operator (compute_T_matrix, src/T_matrix.f90:240-277). */
+int scadyn_op_tmatrix(scadyn_operator *op, int32_t nmax,
+                      double *Taa, double *Tab, double *Tba, double *Tbb,
+                      int32_t *iters);
+
+/* Kernel timing: when enabled, every launch of the hot kernels is bracketed
+ * by HIP events on the operator's stream.  timings[i] = {total ms, launches,
+ * algorithmic bytes} for kernel class i (see SCADYN_KCLASS_*). */
+#define SCADYN_KCLASS_SCATTER  0
+#define SCADYN_KCLASS_FFTX     1
+#define SCADYN_KCLASS_FFTY     2
+#define SCADYN_KCLASS_FFTZCONV 3
+#define SCADYN_KCLASS_IFFTY    4
+#define SCADYN_KCLASS_IFFTX    5
+#define SCADYN_KCLASS_GATHER   6  /* gather + near-zone + combine */
+#define SCADYN_KCLASS_KRYLOV   7
+#define SCADYN_KCLASS_COUNT    8
+int scadyn_op_profile(scadyn_operator *op, int32_t enable);
+int scadyn_op_timings(scadyn_operator *op, double *timings /* [COUNT][3] */);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SCADYN_AMD_H */

@@ -1,0 +1,867 @@
+// Operator lifecycle, batched solve driver and the C ABI (include/scadyn_amd.h).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "fft.hpp"
+#include "internal.hpp"
+#include "krylov.hpp"
+#include "matvec.hpp"
+#include "setup.hpp"
+
+namespace scadyn {
+
+// ---------------------------------------------------------------- errors
+static thread_local std::string g_err;
+void set_error(int, const std::string &msg) { g_err = msg; }
+void fail(int code, const std::string &msg) { throw Error{code, msg}; }
+
+// ---------------------------------------------------------------- device memory
+struct DevMem {
+    std::vector<void *> ptrs;
+    size_t bytes = 0;
+    template <class T>
+    T *alloc(size_t n) {
+        if (n == 0) n = 1;
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, n * sizeof(T));
+        if (e != hipSuccess)
+            fail(SCADYN_ERR_MEMORY, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed: " +
+                                        hipGetErrorString(e));
+        ptrs.push_back(p);
+        bytes += n * sizeof(T);
+        return (T *)p;
+    }
+    template <class T>
+    T *upload(const std::vector<T> &v, hipStream_t s) {
+        T *p = alloc<T>(v.size());
+        // synchronous: the host vector may be a temporary
+        if (!v.empty()) {
+            HIP_CHECK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+        }
+        return p;
+    }
+    void release(void *p) {
+        auto it = std::find(ptrs.begin(), ptrs.end(), p);
+        if (it != ptrs.end()) {
+            (void)hipFree(p);
+            ptrs.erase(it);
+        }
+    }
+    void clear() {
+        for (void *p : ptrs) (void)hipFree(p);
+        ptrs.clear();
+        bytes = 0;
+    }
+    ~DevMem() { clear(); }
+};
+
+// ---------------------------------------------------------------- profiling
+struct Profiler {
+    bool on = false;
+    struct Pending {
+        int cls;
+        hipEvent_t a, b;
+        double bytes;
+    };
+    std::vector<Pending> pend;
+    std::vector<hipEvent_t> pool;
+    double ms[SCADYN_KCLASS_COUNT] = {0}, cnt[SCADYN_KCLASS_COUNT] = {0}, byt[SCADYN_KCLASS_COUNT] = {0};
+    hipEvent_t ev() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        HIP_CHECK(hipEventCreate(&e));
+        return e;
+    }
+    void begin(int cls, double bytes, hipStream_t s, Pending &p) {
+        p.cls = cls;
+        p.bytes = bytes;
+        p.a = ev();
+        p.b = ev();
+        HIP_CHECK(hipEventRecord(p.a, s));
+    }
+    void end(Pending &p, hipStream_t s) {
+        HIP_CHECK(hipEventRecord(p.b, s));
+        pend.push_back(p);
+    }
+    void collect() {
+        for (auto &p : pend) {
+            HIP_CHECK(hipEventSynchronize(p.b));
+            float t = 0;
+            HIP_CHECK(hipEventElapsedTime(&t, p.a, p.b));
+            ms[p.cls] += t;
+            cnt[p.cls] += 1;
+            byt[p.cls] += p.bytes;
+            pool.push_back(p.a);
+            pool.push_back(p.b);
+        }
+        pend.clear();
+    }
+    ~Profiler() {
+        for (auto &p : pend) {
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+// ---------------------------------------------------------------- workspace
+struct Workspace {
+    int B = 0, m = 0;
+    long long n = 0;
+    DevMem mem;
+    cd *grid0 = nullptr, *grid1 = nullptr;
+    cd *V = nullptr, *Rhs = nullptr, *W = nullptr;
+    SlotState *st = nullptr;
+    cd *H = nullptr, *cs = nullptr, *sn = nullptr, *g = nullptr, *y = nullptr;
+    double *nrm2 = nullptr, *nav2 = nullptr, *pnrm = nullptr, *pnav = nullptr;
+    cd *pdot = nullptr;
+    long long *in_off = nullptr;
+    int *col = nullptr, *active = nullptr;
+    int nwg = 0;
+    LinePass px, py, pz, qy, qx;
+};
+
+}  // namespace scadyn
+
+using namespace scadyn;
+
+struct scadyn_operator {
+    int device = 0;
+    hipStream_t s = nullptr;
+    scadyn_opts opts;
+    double k = 0;
+    HostGeometry g;
+    NearPairs np;
+    GridDesc gd;
+    int L[3] = {0, 0, 0}, Hh[3] = {0, 0, 0};
+    DevMem mem;
+    double *d_tcoord = nullptr, *d_vol = nullptr;
+    cd *d_param = nullptr, *d_S4 = nullptr, *d_fg = nullptr, *d_tw = nullptr;
+    int *d_tet_cube = nullptr, *d_cube_start = nullptr, *d_perm = nullptr;
+    cf *d_blocks = nullptr;
+    int64_t *d_row_start = nullptr;
+    int *d_ent_col = nullptr;
+    uint32_t *d_ent_blk = nullptr;
+    long long nblocks = 0;
+    std::unique_ptr<Workspace> ws;
+    Profiler prof;
+    ~scadyn_operator() {
+        ws.reset();
+        mem.clear();
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+namespace scadyn {
+
+static const double PI_H = 3.141592653589793;
+
+static std::vector<cd> twiddle_table(int L) {
+    std::vector<cd> t(L);
+    for (int m = 0; m < L; ++m) {
+        long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)L;
+        t[m] = cmk((double)cosl(a), (double)sinl(a));
+    }
+    return t;
+}
+
+static void fft3d_full(scadyn_operator *op, cd *A, cd *tmp) {
+    // forward unnormalised 3D DFT of a [Lz][Ly][Lx] grid (FFTW_FORWARD sign)
+    const int Lx = op->L[0], Ly = op->L[1], Lz = op->L[2];
+    const size_t budget = 64 * 1024;
+    LinePass p;
+    make_line_pass(p, Lx, Lx, Lx, 1, (long long)Ly * Lz, budget);
+    launch_line_fft(p, A, tmp, op->d_tw, -1, op->s);
+    make_line_pass(p, Ly, Ly, Ly, Lx, Lz, budget);
+    launch_line_fft(p, tmp, A, op->d_tw + Lx, -1, op->s);
+    make_line_pass(p, Lz, Lz, Lz, (long long)Lx * Ly, 1, budget);
+    launch_line_fft(p, A, tmp, op->d_tw + Lx + Ly, -1, op->s);
+    HIP_CHECK(hipMemcpyAsync(A, tmp, sizeof(cd) * (size_t)Lx * Ly * Lz, hipMemcpyDeviceToDevice, op->s));
+}
+
+static void create_operator(scadyn_operator *op, const scadyn_mesh &mesh, const scadyn_opts &opts, double k) {
+    if (!(k > 0.0)) fail(SCADYN_ERR_ARG, "wavenumber must be positive");
+    op->opts = opts;
+    op->k = k;
+    op->device = opts.device;
+    // host-side validation and indexing first: mesh errors surface without a device
+    HostGeometry &g = op->g;
+    build_geometry(mesh, opts, g);
+    build_near_pairs(g, op->np);
+    HIP_CHECK(hipSetDevice(op->device));
+    HIP_CHECK(hipStreamCreateWithFlags(&op->s, hipStreamNonBlocking));
+    hipStream_t s = op->s;
+    op->nblocks = (long long)op->np.T.size();
+    GridDesc &gd = op->gd;
+    for (int c = 0; c < 3; ++c) gd.min[c] = g.min_coord[c];
+    gd.delta = g.delta;
+    gd.box_delta = g.box_delta;
+    gd.Nx = g.Nx;
+    gd.Ny = g.Ny;
+    gd.Nz = g.Nz;
+    gd.ncx = g.ncx;
+    gd.ncy = g.ncy;
+    gd.ncz = g.ncz;
+    const int Nax[3] = {g.Nx, g.Ny, g.Nz};
+    for (int c = 0; c < 3; ++c) {
+        op->L[c] = opts.fft_pad == 1 ? 2 * Nax[c] : smooth_length(2 * Nax[c] - 1);
+        op->Hh[c] = op->L[c] / 2 + 1;
+    }
+    DevMem &M = op->mem;
+    const int N = g.N;
+    op->d_tcoord = M.upload(g.tcoord, s);
+    op->d_vol = M.upload(g.vol, s);
+    op->d_param = M.upload(g.param, s);
+    op->d_tet_cube = M.upload(g.tet_cube, s);
+    op->d_cube_start = M.upload(g.cube_start, s);
+    op->d_perm = M.upload(g.perm, s);
+    {
+        std::vector<cd> tw;
+        for (int c = 0; c < 3; ++c) {
+            auto t = twiddle_table(op->L[c]);
+            tw.insert(tw.end(), t.begin(), t.end());
+        }
+        op->d_tw = M.upload(tw, s);
+    }
+    // ---- W (98 x 27) from the first tet's cube and its pseudo-inverse
+    //      (src/projection.f90:71-101)
+    std::vector<double> P0;
+    {
+        const double gv[5] = {-1.0, -0.5, 0.0, 0.5, 1.0};
+        for (int z = 0; z < 5; ++z)
+            for (int y = 0; y < 5; ++y)
+                for (int x = 0; x < 5; ++x)
+                    if (std::max(std::fabs(gv[x]), std::max(std::fabs(gv[y]), std::fabs(gv[z]))) == 1.0) {
+                        P0.push_back(gv[x]);
+                        P0.push_back(gv[y]);
+                        P0.push_back(gv[z]);
+                    }
+    }
+    std::vector<double> Psph(98 * 3);
+    for (int i = 0; i < 98 * 3; ++i) Psph[i] = 2.0 * g.box_delta * P0[i];
+    {
+        const int c0 = g.tet_cube[g.inv[0]];
+        const int cx = c0 % g.ncx, cy = (c0 / g.ncx) % g.ncy, cz = c0 / (g.ncx * g.ncy);
+        const int base[3] = {2 * cx, 2 * cy, 2 * cz};
+        double node1[3], cp[3];
+        for (int c = 0; c < 3; ++c) {
+            node1[c] = g.min_coord[c] + g.delta * base[c];
+            cp[c] = node1[c] + g.box_delta / 2.0;
+        }
+        std::vector<cd> W(98 * 27);
+        for (int u = 0; u < 27; ++u) {
+            const int o[3] = {u % 3, (u / 3) % 3, u / 9};
+            double pn[3];
+            for (int c = 0; c < 3; ++c) pn[c] = g.min_coord[c] + g.delta * (base[c] + o[c]);
+            for (int m = 0; m < 98; ++m) {
+                double d2 = 0;
+                for (int c = 0; c < 3; ++c) {
+                    double a = (Psph[3 * m + c] + cp[c]) - pn[c];
+                    d2 += a * a;
+                }
+                double R = std::sqrt(d2), den = 4.0 * PI_H * R;
+                W[(size_t)u * 98 + m] = cmk(std::cos(k * R) / den, std::sin(k * R) / den);
+            }
+        }
+        std::vector<cd> WW = pinv_jacobi(W, 98, 27, 1e-6);
+        cd *d_WW = M.upload(WW, s);
+        upload_tables(Psph.data(), s);
+        op->d_S4 = M.alloc<cd>((size_t)N * 108);
+        launch_projection(op->d_tcoord, op->d_vol, op->d_tet_cube, gd, k, d_WW, op->d_S4, N, s);
+        HIP_CHECK(hipStreamSynchronize(s));
+        M.release(d_WW);
+    }
+    // ---- near zone
+    {
+        cd *d_G = M.alloc<cd>(27 * 27 * 27);
+        launch_pg_table(d_G, gd, k, s);
+        int *d_PT = M.upload(op->np.T, s);
+        int *d_PB = M.upload(op->np.B, s);
+        uint8_t *d_PO = M.upload(op->np.off, s);
+        op->d_blocks = M.alloc<cf>((size_t)op->nblocks * 9);
+        launch_near_blocks(d_PT, d_PB, d_PO, op->nblocks, op->d_tcoord, op->d_vol, op->d_param, op->d_S4, d_G, k,
+                           op->d_blocks, s);
+        op->d_row_start = M.upload(op->np.row_start, s);
+        op->d_ent_col = M.upload(op->np.ent_col, s);
+        op->d_ent_blk = M.upload(op->np.ent_blk, s);
+        HIP_CHECK(hipStreamSynchronize(s));
+        M.release(d_PT);
+        M.release(d_PB);
+        M.release(d_PO);
+        M.release(d_G);
+        std::vector<int>().swap(op->np.ent_col);
+        std::vector<uint32_t>().swap(op->np.ent_blk);
+        std::vector<uint8_t>().swap(op->np.off);
+    }
+    // ---- FFT(G) (src/build_G.f90)
+    {
+        const size_t tot = (size_t)op->L[0] * op->L[1] * op->L[2];
+        cd *A = M.alloc<cd>(tot), *T = M.alloc<cd>(tot);
+        launch_green_grid(A, gd, op->L[0], op->L[1], op->L[2], k, s);
+        fft3d_full(op, A, T);
+        op->d_fg = M.alloc<cd>((size_t)op->Hh[0] * op->Hh[1] * op->Hh[2]);
+        launch_fg_octant(A, op->d_fg, op->L[0], op->L[1], op->L[2], s);
+        HIP_CHECK(hipStreamSynchronize(s));
+        M.release(A);
+        M.release(T);
+    }
+}
+
+// ---------------------------------------------------------------- workspace / matvec
+static size_t slot_bytes(const scadyn_operator *op, int m) {
+    const size_t n = 3 * (size_t)op->g.N;
+    const size_t grid = (size_t)op->L[0] * op->L[1] * op->g.Nz * 4;
+    return sizeof(cd) * ((size_t)(m + 4) * n + 2 * grid);
+}
+
+static Workspace &workspace(scadyn_operator *op, int B) {
+    const int m = op->opts.maxit;
+    if (op->ws && op->ws->B == B && op->ws->m == m) return *op->ws;
+    op->ws.reset(new Workspace());
+    Workspace &w = *op->ws;
+    w.B = B;
+    w.m = m;
+    w.n = 3LL * op->g.N;
+    const size_t nB = (size_t)w.n * B;
+    const int Lx = op->L[0], Ly = op->L[1], Lz = op->L[2];
+    const int Nx = op->g.Nx, Ny = op->g.Ny, Nz = op->g.Nz;
+    const long long nb = 4LL * B;
+    const size_t gsz = (size_t)Lx * Ly * Nz * nb;
+    w.grid0 = w.mem.alloc<cd>(gsz);
+    w.grid1 = w.mem.alloc<cd>(gsz);
+    w.V = w.mem.alloc<cd>((size_t)(m + 2) * nB);
+    w.Rhs = w.mem.alloc<cd>(nB);
+    w.W = w.mem.alloc<cd>(nB);
+    w.st = w.mem.alloc<SlotState>(B);
+    HIP_CHECK(hipMemsetAsync(w.st, 0, sizeof(SlotState) * B, op->s));
+    w.H = w.mem.alloc<cd>((size_t)B * (m + 1) * m);
+    w.cs = w.mem.alloc<cd>((size_t)B * (m + 1));
+    w.sn = w.mem.alloc<cd>((size_t)B * (m + 1));
+    w.g = w.mem.alloc<cd>((size_t)B * (m + 1));
+    w.y = w.mem.alloc<cd>((size_t)B * m);
+    w.nwg = 512;
+    w.nrm2 = w.mem.alloc<double>(B);
+    w.nav2 = w.mem.alloc<double>(B);
+    w.pnrm = w.mem.alloc<double>((size_t)w.nwg * B);
+    w.pnav = w.mem.alloc<double>((size_t)w.nwg * B);
+    w.pdot = w.mem.alloc<cd>((size_t)w.nwg * B);
+    w.in_off = w.mem.alloc<long long>(B);
+    w.col = w.mem.alloc<int>(B);
+    w.active = w.mem.alloc<int>(B);
+    const size_t budget = 64 * 1024;
+    make_line_pass(w.px, Lx, Nx, Lx, nb, (long long)Ny * Nz, budget);
+    make_line_pass(w.py, Ly, Ny, Ly, (long long)Lx * nb, Nz, budget);
+    make_line_pass(w.pz, Lz, Nz, Nz, (long long)Ly * Lx * nb, 1, budget);
+    make_line_pass(w.qy, Ly, Ly, Ny, (long long)Lx * nb, Nz, budget);
+    make_line_pass(w.qx, Lx, Lx, Nx, nb, (long long)Ny * Nz, budget);
+    return w;
+}
+
+struct ProfScope {
+    scadyn_operator *op;
+    Profiler::Pending p;
+    bool on;
+    ProfScope(scadyn_operator *o, int cls, double bytes) : op(o), on(o->prof.on) {
+        if (on) op->prof.begin(cls, bytes, op->s, p);
+    }
+    ~ProfScope() {
+        if (on) op->prof.end(p, op->s);
+    }
+};
+
+// y = A x for the batch: x element (row, b) at xin[in_off[b] + row*B + b]
+static void run_matvec(scadyn_operator *op, Workspace &w, const cd *xin, const int *active, cd *y) {
+    const HostGeometry &g = op->g;
+    MatvecArgs a;
+    a.xin = xin;
+    a.in_off = w.in_off;
+    a.B = w.B;
+    a.N = g.N;
+    a.Nx = g.Nx;
+    a.Ny = g.Ny;
+    a.Nz = g.Nz;
+    a.ncx = g.ncx;
+    a.ncy = g.ncy;
+    a.ncz = g.ncz;
+    a.cube_start = op->d_cube_start;
+    a.tet_cube = op->d_tet_cube;
+    a.S4 = op->d_S4;
+    a.row_start = op->d_row_start;
+    a.ent_col = op->d_ent_col;
+    a.ent_blk = op->d_ent_blk;
+    a.blocks = op->d_blocks;
+    a.k2 = op->k * op->k;
+    a.slot_active = active;
+    a.grid_in = w.grid0;
+    a.grid_out = w.grid0;
+    a.y = y;
+    const double c = sizeof(cd);
+    const double nB = 4.0 * w.B;
+    const double Lx = op->L[0], Ly = op->L[1], Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+    {
+        ProfScope ps(op, SCADYN_KCLASS_SCATTER, c * (Nx * Ny * Nz * nB + 3.0 * g.N * w.B) + 1728.0 * g.N);
+        launch_scatter(a, op->s);
+    }
+    {
+        ProfScope ps(op, SCADYN_KCLASS_FFTX, c * nB * (Nx * Ny * Nz + Lx * Ny * Nz));
+        launch_line_fft(w.px, w.grid0, w.grid1, op->d_tw, -1, op->s);
+    }
+    {
+        ProfScope ps(op, SCADYN_KCLASS_FFTY, c * nB * (Lx * Ny * Nz + Lx * Ly * Nz));
+        launch_line_fft(w.py, w.grid1, w.grid0, op->d_tw + op->L[0], -1, op->s);
+    }
+    {
+        ConvInfo ci;
+        ci.fg = op->d_fg;
+        ci.Lx = op->L[0];
+        ci.Ly = op->L[1];
+        ci.Lz = op->L[2];
+        ci.Hx = op->Hh[0];
+        ci.Hy = op->Hh[1];
+        ci.nb = 4LL * w.B;
+        ProfScope ps(op, SCADYN_KCLASS_FFTZCONV,
+                     c * (nB * 2.0 * Lx * Ly * Nz + (double)op->Hh[0] * op->Hh[1] * op->Hh[2]));
+        launch_line_conv(w.pz, w.grid0, w.grid0, op->d_tw + op->L[0] + op->L[1], ci, op->s);
+    }
+    {
+        ProfScope ps(op, SCADYN_KCLASS_IFFTY, c * nB * (Lx * Ly * Nz + Lx * Ny * Nz));
+        launch_line_fft(w.qy, w.grid0, w.grid1, op->d_tw + op->L[0], +1, op->s);
+    }
+    {
+        ProfScope ps(op, SCADYN_KCLASS_IFFTX, c * nB * (Lx * Ny * Nz + Nx * Ny * Nz));
+        launch_line_fft(w.qx, w.grid1, w.grid0, op->d_tw, +1, op->s);
+    }
+    {
+        ProfScope ps(op, SCADYN_KCLASS_GATHER,
+                     c * (Nx * Ny * Nz * nB + 6.0 * g.N * w.B) + 1728.0 * g.N + 80.0 * op->nblocks);
+        launch_gather_near(a, op->s);
+    }
+}
+
+// pack/unpack between host order [b][3N] (reference tets) and [row][B] (internal)
+__global__ void k_pack(const cd *__restrict__ src, int nrhs, int N, const int *__restrict__ perm, int B,
+                       cd *__restrict__ dst) {
+    const long long tot = 3LL * N * B;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int b = (int)(e % B);
+        const long long row = e / B;
+        const int i = (int)(row / 3), c = (int)(row % 3);
+        dst[e] = b < nrhs ? src[(size_t)b * 3 * N + 3 * (size_t)perm[i] + c] : cmk(0, 0);
+    }
+}
+__global__ void k_unpack(const cd *__restrict__ src, int nrhs, int N, const int *__restrict__ perm, int B,
+                         cd *__restrict__ dst) {
+    const long long tot = 3LL * N * B;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int b = (int)(e % B);
+        if (b >= nrhs) continue;
+        const long long row = e / B;
+        const int i = (int)(row / 3), c = (int)(row % 3);
+        dst[(size_t)b * 3 * N + 3 * (size_t)perm[i] + c] = src[e];
+    }
+}
+static unsigned nblk(long long tot) { return (unsigned)std::min<long long>((tot + 255) / 256, 1 << 16); }
+
+// host -> device copy of a host buffer that is reused right after the call
+static void h2d_sync(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+}
+
+static int auto_batch(scadyn_operator *op, int ncols) {
+    if (op->opts.batch > 0) return std::min(op->opts.batch, std::max(ncols, 1));
+    size_t free_b = 0, total_b = 0;
+    HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+    size_t per = slot_bytes(op, op->opts.maxit);
+    size_t cap = (size_t)(0.6 * (double)free_b) / std::max<size_t>(per, 1);
+    int B = (int)std::min<size_t>(cap, 64);
+    B = std::max(1, std::min(B, ncols));
+    if (cap == 0) fail(SCADYN_ERR_MEMORY, "not enough device memory for one RHS slot");
+    return B;
+}
+
+// Solve for the columns of src (n x ld, internal rows); solutions into dst.
+static void solve_columns(scadyn_operator *op, const cd *src, long long ld, int ncols, cd *dst, int32_t *iters,
+                          int *nconv_fail) {
+    const int B = auto_batch(op, ncols);
+    Workspace &w = workspace(op, B);
+    hipStream_t s = op->s;
+    KrylovDims d;
+    d.n = w.n;
+    d.B = B;
+    d.m = w.m;
+    d.nwg = w.nwg;
+    const size_t nB = (size_t)w.n * B;
+    std::vector<SlotState> hs(B);
+    std::vector<int> slot_col(B, -1), loadc(B, -1), act(B, 0);
+    std::vector<long long> off(B, 0);
+    int next = 0, nfail = 0, singular = 0;
+    HIP_CHECK(hipMemsetAsync(w.st, 0, sizeof(SlotState) * B, s));
+    HIP_CHECK(hipMemsetAsync(w.V, 0, sizeof(cd) * (size_t)(w.m + 2) * nB, s));
+    // every slot finishes within restart*(m+1) matvecs; bound the loop
+    const long long max_steps =
+        ((long long)op->opts.restart * (w.m + 1) + 4) * ((ncols + B - 1) / B + 2) + 16;
+    long long steps = 0;
+    for (;;) {
+        if (++steps > max_steps) fail(SCADYN_ERR_DEVICE, "GMRES driver made no progress");
+        // refill free slots
+        bool any_load = false;
+        for (int b = 0; b < B; ++b) {
+            loadc[b] = -1;
+            if (slot_col[b] < 0 && next < ncols) {
+                loadc[b] = next++;
+                slot_col[b] = loadc[b];
+                any_load = true;
+            }
+        }
+        if (any_load) {
+            h2d_sync(w.col, loadc.data(), sizeof(int) * B, s);
+            launch_load_rhs(d, w.col, src, ld, w.Rhs, w.V, w.pnrm, s);
+            launch_load_finish(d, w.col, w.st, w.pnrm, s);
+        }
+        HIP_CHECK(hipMemcpyAsync(hs.data(), w.st, sizeof(SlotState) * B, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        // slots finished at load (zero rhs)
+        bool busy = false;
+        int K = -1;
+        for (int b = 0; b < B; ++b) {
+            act[b] = 0;
+            off[b] = 0;
+            if (slot_col[b] < 0) continue;
+            if (hs[b].phase == PH_RESTART) {
+                act[b] = 1;
+                off[b] = (long long)(w.m + 1) * nB;
+                busy = true;
+                K = std::max(K, 0);
+            } else if (hs[b].phase == PH_ARNOLDI) {
+                act[b] = 1;
+                off[b] = (long long)hs[b].i * nB;
+                busy = true;
+                K = std::max(K, hs[b].i + 1);
+            }
+        }
+        if (busy) {
+            h2d_sync(w.in_off, off.data(), sizeof(long long) * B, s);
+            h2d_sync(w.active, act.data(), sizeof(int) * B, s);
+            run_matvec(op, w, w.V, w.active, w.W);
+            {
+                ProfScope ps(op, SCADYN_KCLASS_KRYLOV, 0.0);
+                for (int kk = 0; kk <= K; ++kk) {
+                    launch_mgs_pass(d, kk, w.W, w.V, w.Rhs, w.st, w.H, w.pdot, w.pnrm, w.pnav, s);
+                    launch_mgs_reduce(d, kk, w.st, w.H, w.pdot, w.pnrm, w.pnav, w.nrm2, w.nav2, s);
+                }
+                launch_slot_update(d, w.st, w.H, w.cs, w.sn, w.g, w.y, w.nrm2, w.nav2, op->opts.tol,
+                                   op->opts.restart, s);
+                launch_vec_update(d, w.st, w.W, w.V, w.y, s);
+            }
+            HIP_CHECK(hipMemcpyAsync(hs.data(), w.st, sizeof(SlotState) * B, hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+            if (op->prof.on) op->prof.collect();
+        }
+        // retire finished slots
+        bool any_done = false;
+        for (int b = 0; b < B; ++b) {
+            loadc[b] = -1;
+            if (slot_col[b] >= 0 && hs[b].phase == PH_DONE) {
+                loadc[b] = slot_col[b];
+                if (iters) iters[slot_col[b]] = hs[b].ite;
+                if (hs[b].conv == 0) nfail++;
+                if (hs[b].conv < 0) singular++;
+                slot_col[b] = -1;
+                any_done = true;
+            }
+        }
+        if (any_done) {
+            h2d_sync(w.col, loadc.data(), sizeof(int) * B, s);
+            launch_store_x(d, w.col, w.V, dst, ld, s);
+            // mark retired slots idle
+            for (int b = 0; b < B; ++b)
+                if (loadc[b] >= 0) hs[b].phase = PH_IDLE;
+            h2d_sync(w.st, hs.data(), sizeof(SlotState) * B, s);
+        }
+        bool all_idle = true;
+        for (int b = 0; b < B; ++b)
+            if (slot_col[b] >= 0) all_idle = false;
+        if (all_idle && next >= ncols) break;
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (singular) fail(SCADYN_ERR_SINGULAR, "Matrix is numerically singular! (GMRES least squares)");
+    if (nconv_fail) *nconv_fail = nfail;
+}
+
+static void tmatrix(scadyn_operator *op, int nmax, double *Taa, double *Tab, double *Tba, double *Tbb,
+                    int32_t *iters) {
+    if (nmax < 1) fail(SCADYN_ERR_ARG, "nmax must be >= 1");
+    const int nm = (nmax + 1) * (nmax + 1) - 1, ncol = 2 * nm;
+    const long long n = 3LL * op->g.N;
+    hipStream_t s = op->s;
+    DevMem tmp;
+    cd *mat = tmp.alloc<cd>((size_t)n * ncol);
+    launch_vswf(op->d_tcoord, op->d_vol, op->g.N, nmax, op->k, mat, ncol, s);
+    cd *X = tmp.alloc<cd>((size_t)n * ncol);
+    int nfail = 0;
+    solve_columns(op, mat, ncol, ncol, X, iters, &nfail);
+    const size_t work = std::max<size_t>((size_t)ncol * ncol * 64, 1 << 20);
+    cd *wk = tmp.alloc<cd>(work);
+    cd *T = tmp.alloc<cd>((size_t)ncol * ncol);
+    launch_tgemm(mat, X, n, ncol, T, wk, work, s);
+    std::vector<cd> hT((size_t)ncol * ncol);
+    HIP_CHECK(hipMemcpyAsync(hT.data(), T, sizeof(cd) * hT.size(), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    // sc = i k^3; Taa = T(1:nm,1:nm)*sc etc., column-major outputs (src/T_matrix.f90:270-275)
+    const double k3 = std::pow(op->k, 3.0);
+    double *outs[4] = {Taa, Tab, Tba, Tbb};
+    for (int q = 0; q < 4; ++q) {
+        const int r0 = (q >= 2) ? nm : 0, c0 = (q % 2) ? nm : 0;
+        for (int j = 0; j < nm; ++j)
+            for (int i = 0; i < nm; ++i) {
+                cd v = hT[(size_t)(r0 + i) * ncol + (c0 + j)];
+                // (v.x + i v.y) * (0 + i k3) = (-v.y k3) + i (v.x k3)
+                outs[q][2 * ((size_t)j * nm + i)] = -v.y * k3;
+                outs[q][2 * ((size_t)j * nm + i) + 1] = v.x * k3;
+            }
+    }
+    if (nfail && op->opts.strict)
+        fail(SCADYN_ERR_NOCONV, std::to_string(nfail) + " T-matrix columns did not reach tol");
+}
+
+}  // namespace scadyn
+
+// ================================================================ C ABI
+#define ABI_TRY try {
+#define ABI_CATCH                                                   \
+    }                                                               \
+    catch (const scadyn::Error &e) {                                \
+        scadyn::set_error(e.code, e.msg);                           \
+        return e.code;                                              \
+    }                                                               \
+    catch (const std::exception &e) {                               \
+        scadyn::set_error(SCADYN_ERR_DEVICE, e.what());             \
+        return SCADYN_ERR_DEVICE;                                   \
+    }                                                               \
+    return SCADYN_OK;
+
+extern "C" {
+
+void scadyn_opts_default(scadyn_opts *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->tol = 1e-5;
+    o->maxit = 50;
+    o->restart = 4;
+    o->M_ex = 2;
+    o->near_zone = 1;
+    o->grid_size = 1.8;
+    o->order = 0;
+    o->precision = SCADYN_PREC_FP64;
+    o->batch = 0;
+    o->strict = 0;
+    o->device = 0;
+    o->fft_pad = 0;
+    o->refine_tol = 1e-9;
+}
+
+const char *scadyn_last_error(void) { return g_err.c_str(); }
+
+int scadyn_op_create(const scadyn_mesh *mesh, const scadyn_opts *opts, double k, scadyn_operator **out) {
+    ABI_TRY
+    if (!mesh || !out) fail(SCADYN_ERR_ARG, "null argument");
+    scadyn_opts o;
+    if (opts)
+        o = *opts;
+    else
+        scadyn_opts_default(&o);
+    if (o.precision != SCADYN_PREC_FP64) fail(SCADYN_ERR_UNSUPPORTED, "only the FP64 path is built");
+    if (o.maxit < 1 || o.restart < 1) fail(SCADYN_ERR_ARG, "maxit and restart must be >= 1");
+    *out = nullptr;
+    std::unique_ptr<scadyn_operator> op(new scadyn_operator());
+    create_operator(op.get(), *mesh, o, k);
+    *out = op.release();
+    ABI_CATCH
+}
+
+void scadyn_op_destroy(scadyn_operator *op) {
+    if (!op) return;
+    (void)hipSetDevice(op->device);
+    delete op;
+}
+
+int scadyn_op_info(const scadyn_operator *op, int64_t info[16], double dinfo[4]) {
+    ABI_TRY
+    if (!op) fail(SCADYN_ERR_ARG, "null operator");
+    if (info) {
+        for (int i = 0; i < 16; ++i) info[i] = 0;
+        info[0] = op->g.N;
+        info[1] = op->g.Nx;
+        info[2] = op->g.Ny;
+        info[3] = op->g.Nz;
+        info[4] = op->g.N_cubes;
+        info[5] = op->g.N_tet_cube;
+        info[6] = op->nblocks;
+        info[7] = op->L[0];
+        info[8] = op->L[1];
+        info[9] = op->L[2];
+        info[10] = op->ws ? op->ws->B : 0;
+        info[11] = (int64_t)(op->mem.bytes + (op->ws ? op->ws->mem.bytes : 0));
+    }
+    if (dinfo) {
+        dinfo[0] = op->g.delta;
+        dinfo[1] = op->g.box_delta;
+        dinfo[2] = op->k;
+        dinfo[3] = 0;
+    }
+    ABI_CATCH
+}
+
+int scadyn_op_export(const scadyn_operator *opc, int32_t what, void *host, int64_t nbytes) {
+    ABI_TRY
+    scadyn_operator *op = const_cast<scadyn_operator *>(opc);
+    if (!op || !host) fail(SCADYN_ERR_ARG, "null argument");
+    HIP_CHECK(hipSetDevice(op->device));
+    const int N = op->g.N;
+    if (what == SCADYN_EXPORT_S) {
+        if (nbytes != (int64_t)sizeof(cd) * 4 * N * 27) fail(SCADYN_ERR_ARG, "size mismatch");
+        std::vector<cd> s4((size_t)N * 108);
+        HIP_CHECK(hipMemcpy(s4.data(), op->d_S4, sizeof(cd) * s4.size(), hipMemcpyDeviceToHost));
+        cd *o = (cd *)host;
+        for (int ch = 0; ch < 4; ++ch)
+            for (int t = 0; t < N; ++t)
+                for (int j = 0; j < 27; ++j)
+                    o[((size_t)ch * N + t) * 27 + j] = s4[((size_t)op->g.inv[t] * 27 + j) * 4 + ch];
+    } else if (what == SCADYN_EXPORT_BLOCKS) {
+        if (nbytes != (int64_t)sizeof(cf) * 9 * op->nblocks) fail(SCADYN_ERR_ARG, "size mismatch");
+        HIP_CHECK(hipMemcpy(host, op->d_blocks, nbytes, hipMemcpyDeviceToHost));
+    } else if (what == SCADYN_EXPORT_BLKIND) {
+        if (nbytes != (int64_t)sizeof(int32_t) * 2 * op->nblocks) fail(SCADYN_ERR_ARG, "size mismatch");
+        int32_t *o = (int32_t *)host;
+        for (long long p = 0; p < op->nblocks; ++p) {
+            o[2 * p] = op->g.perm[op->np.T[p]] + 1;
+            o[2 * p + 1] = op->g.perm[op->np.B[p]] + 1;
+        }
+    } else if (what == SCADYN_EXPORT_FG) {
+        size_t n = (size_t)op->Hh[0] * op->Hh[1] * op->Hh[2];
+        if (nbytes != (int64_t)(sizeof(cd) * n)) fail(SCADYN_ERR_ARG, "size mismatch");
+        HIP_CHECK(hipMemcpy(host, op->d_fg, nbytes, hipMemcpyDeviceToHost));
+    } else {
+        fail(SCADYN_ERR_ARG, "unknown export id");
+    }
+    ABI_CATCH
+}
+
+int scadyn_op_matvec(scadyn_operator *op, int32_t nrhs, const double *x, double *y) {
+    ABI_TRY
+    if (!op || !x || !y || nrhs < 1) fail(SCADYN_ERR_ARG, "bad argument");
+    HIP_CHECK(hipSetDevice(op->device));
+    const int N = op->g.N;
+    Workspace &w = workspace(op, nrhs);
+    hipStream_t s = op->s;
+    DevMem tmp;
+    const size_t nv = (size_t)nrhs * 3 * N;
+    cd *hx = tmp.alloc<cd>(nv);
+    cd *xin = tmp.alloc<cd>(nv);
+    HIP_CHECK(hipMemcpyAsync(hx, x, sizeof(cd) * nv, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_pack, dim3(nblk(3LL * N * nrhs)), dim3(256), 0, s, hx, nrhs, N, op->d_perm, nrhs, xin);
+    std::vector<long long> off(nrhs, 0);
+    h2d_sync(w.in_off, off.data(), sizeof(long long) * nrhs, s);
+    run_matvec(op, w, xin, nullptr, w.W);
+    hipLaunchKernelGGL(k_unpack, dim3(nblk(3LL * N * nrhs)), dim3(256), 0, s, w.W, nrhs, N, op->d_perm, nrhs, hx);
+    HIP_CHECK(hipMemcpyAsync(y, hx, sizeof(cd) * nv, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (op->prof.on) op->prof.collect();
+    ABI_CATCH
+}
+
+int scadyn_op_solve(scadyn_operator *op, int32_t nrhs, const double *rhs, double *x, int32_t *iters) {
+    ABI_TRY
+    if (!op || !rhs || !x || nrhs < 1) fail(SCADYN_ERR_ARG, "bad argument");
+    HIP_CHECK(hipSetDevice(op->device));
+    const int N = op->g.N;
+    hipStream_t s = op->s;
+    DevMem tmp;
+    const size_t nv = (size_t)nrhs * 3 * N;
+    cd *hx = tmp.alloc<cd>(nv);
+    cd *src = tmp.alloc<cd>(nv);
+    cd *dst = tmp.alloc<cd>(nv);
+    HIP_CHECK(hipMemcpyAsync(hx, rhs, sizeof(cd) * nv, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_pack, dim3(nblk(3LL * N * nrhs)), dim3(256), 0, s, hx, nrhs, N, op->d_perm, nrhs, src);
+    int nfail = 0;
+    solve_columns(op, src, nrhs, nrhs, dst, iters, &nfail);
+    hipLaunchKernelGGL(k_unpack, dim3(nblk(3LL * N * nrhs)), dim3(256), 0, s, dst, nrhs, N, op->d_perm, nrhs, hx);
+    HIP_CHECK(hipMemcpyAsync(x, hx, sizeof(cd) * nv, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (nfail && op->opts.strict) fail(SCADYN_ERR_NOCONV, std::to_string(nfail) + " systems did not reach tol");
+    ABI_CATCH
+}
+
+int scadyn_op_vswf(scadyn_operator *op, int32_t nmax, double *mat) {
+    ABI_TRY
+    if (!op || !mat || nmax < 1) fail(SCADYN_ERR_ARG, "bad argument");
+    HIP_CHECK(hipSetDevice(op->device));
+    const int N = op->g.N;
+    const int nm = (nmax + 1) * (nmax + 1) - 1, ncol = 2 * nm;
+    DevMem tmp;
+    cd *d = tmp.alloc<cd>((size_t)3 * N * ncol);
+    launch_vswf(op->d_tcoord, op->d_vol, N, nmax, op->k, d, ncol, op->s);
+    std::vector<cd> h((size_t)3 * N * ncol);
+    HIP_CHECK(hipMemcpyAsync(h.data(), d, sizeof(cd) * h.size(), hipMemcpyDeviceToHost, op->s));
+    HIP_CHECK(hipStreamSynchronize(op->s));
+    cd *o = (cd *)mat;
+    const size_t n = 3 * (size_t)N;
+    for (int i = 0; i < N; ++i)
+        for (int c = 0; c < 3; ++c)
+            for (int j = 0; j < ncol; ++j)
+                o[(size_t)j * n + 3 * (size_t)op->g.perm[i] + c] = h[(size_t)(3 * i + c) * ncol + j];
+    ABI_CATCH
+}
+
+int scadyn_op_tmatrix(scadyn_operator *op, int32_t nmax, double *Taa, double *Tab, double *Tba, double *Tbb,
+                      int32_t *iters) {
+    ABI_TRY
+    if (!op || !Taa || !Tab || !Tba || !Tbb) fail(SCADYN_ERR_ARG, "null argument");
+    HIP_CHECK(hipSetDevice(op->device));
+    tmatrix(op, nmax, Taa, Tab, Tba, Tbb, iters);
+    ABI_CATCH
+}
+
+int scadyn_tmat_solve(const scadyn_mesh *mesh, const scadyn_opts *opts, double k, int32_t nmax, double *Taa,
+                      double *Tab, double *Tba, double *Tbb, int32_t *iters) {
+    scadyn_operator *op = nullptr;
+    int rc = scadyn_op_create(mesh, opts, k, &op);
+    if (rc) return rc;
+    rc = scadyn_op_tmatrix(op, nmax, Taa, Tab, Tba, Tbb, iters);
+    scadyn_op_destroy(op);
+    return rc;
+}
+
+int scadyn_op_profile(scadyn_operator *op, int32_t enable) {
+    ABI_TRY
+    if (!op) fail(SCADYN_ERR_ARG, "null operator");
+    op->prof.on = enable != 0;
+    for (int i = 0; i < SCADYN_KCLASS_COUNT; ++i) op->prof.ms[i] = op->prof.cnt[i] = op->prof.byt[i] = 0;
+    ABI_CATCH
+}
+
+int scadyn_op_timings(scadyn_operator *op, double *t) {
+    ABI_TRY
+    if (!op || !t) fail(SCADYN_ERR_ARG, "null argument");
+    op->prof.collect();
+    for (int i = 0; i < SCADYN_KCLASS_COUNT; ++i) {
+        t[3 * i] = op->prof.ms[i];
+        t[3 * i + 1] = op->prof.cnt[i];
+        t[3 * i + 2] = op->prof.byt[i];
+    }
+    ABI_CATCH
+}
+
+}  // extern "C"

@@ -1,0 +1,50 @@
+// Batched strided line FFTs in LDS (Stockham autosort, mixed radix).
+//
+// One pass transforms lines along one axis of a grid laid out as
+// [outer][axis][inner] (inner contiguous).  A workgroup owns a tile of `ti`
+// consecutive inner columns of one outer slab and all L points of the line,
+// so global loads/stores are coalesced along the inner (contiguous) index.
+// The input line has n_in valid points (the rest are the zero padding of the
+// pruned transform, src/possu.f90:140-204) and only the first n_out outputs
+// are stored (the crop of src/possu.f90:225-289 / arr2vec).
+#pragma once
+
+#include "internal.hpp"
+
+namespace scadyn {
+
+struct LinePass {
+    int L = 0, n_in = 0, n_out = 0;
+    int nrad = 0;
+    int rad[16] = {0};
+    long long inner = 0;  // contiguous inner extent (elements)
+    long long outer = 0;  // number of outer slabs
+    int ti = 0;           // tile width (inner columns per workgroup)
+};
+
+// Fg lookup for the fused z-pass: inner index i of the [Ly][Lx][nb] layout
+// -> frequency (kx, ky); kz is the line index.  Fg holds the even octant.
+struct ConvInfo {
+    const cd *fg = nullptr;
+    int Lx = 0, Ly = 0, Lz = 0, Hx = 0, Hy = 0;
+    long long nb = 0;  // batch columns per grid point
+};
+
+// twiddle tables for lengths used by one operator (device memory)
+struct Twiddles {
+    cd *dev = nullptr;   // concatenated tables exp(-2 pi i m / L), m < L
+    int offset[3] = {0, 0, 0};
+};
+
+void make_line_pass(LinePass &p, int L, int n_in, int n_out, long long inner, long long outer,
+                    size_t lds_budget);
+size_t line_pass_lds(const LinePass &p);
+
+// mode 0: forward (sign -1) or inverse (sign +1) transform
+void launch_line_fft(const LinePass &p, const cd *in, cd *out, const cd *tw, int sign,
+                     hipStream_t s);
+// mode 1: forward, multiply by Fg, inverse (the z-pass of the convolution)
+void launch_line_conv(const LinePass &p, const cd *in, cd *out, const cd *tw,
+                      const ConvInfo &ci, hipStream_t s);
+
+}  // namespace scadyn

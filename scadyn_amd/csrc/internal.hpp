@@ -1,0 +1,102 @@
+// Internal declarations of the MI355X JVIE T-matrix library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/scadyn_amd.h"
+
+namespace scadyn {
+
+// ---------------------------------------------------------------- errors
+struct Error {
+    int code;
+    std::string msg;
+};
+void set_error(int code, const std::string &msg);
+[[noreturn]] void fail(int code, const std::string &msg);
+
+#define HIP_CHECK(expr)                                                              \
+    do {                                                                             \
+        hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess)                                                        \
+            ::scadyn::fail(SCADYN_ERR_DEVICE, std::string(#expr) + ": " +            \
+                                                  hipGetErrorString(e_));            \
+    } while (0)
+
+// ---------------------------------------------------------------- complex
+typedef double2 cd;
+typedef float2 cf;
+
+__host__ __device__ inline cd cmk(double r, double i) { return make_double2(r, i); }
+__host__ __device__ inline cd cadd(cd a, cd b) { return make_double2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline cd csub(cd a, cd b) { return make_double2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline cd cmul(cd a, cd b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// conj(a) * b
+__host__ __device__ inline cd cmulc(cd a, cd b) {
+    return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__host__ __device__ inline cd cscale(cd a, double s) { return make_double2(a.x * s, a.y * s); }
+__host__ __device__ inline cd cconj(cd a) { return make_double2(a.x, -a.y); }
+__host__ __device__ inline double cabs2(cd a) { return a.x * a.x + a.y * a.y; }
+__host__ __device__ inline cd cdiv(cd a, cd b) {
+    // plain formula a * conj(b) / |b|^2, as compilers emit for complex(8) division
+    double d = b.x * b.x + b.y * b.y;
+    return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+// acc += a * b
+__host__ __device__ inline void cfma(cd &acc, cd a, cd b) {
+    acc.x = fma(a.x, b.x, fma(-a.y, b.y, acc.x));
+    acc.y = fma(a.x, b.y, fma(a.y, b.x, acc.y));
+}
+
+// ---------------------------------------------------------------- geometry (host)
+struct HostGeometry {
+    int N = 0, Nnode = 0;
+    int Nx = 0, Ny = 0, Nz = 0;             // grid nodes per axis (src/geometry.f90:90-98)
+    int ncx = 0, ncy = 0, ncz = 0;          // cubes per axis (src/geometry.f90:211-217)
+    int N_cubes = 0, N_tet_cube = 0;
+    double delta = 0, box_delta = 0;
+    double min_coord[3] = {0, 0, 0};
+    std::vector<int> perm;        // internal tet i -> reference tet perm[i] (0-based)
+    std::vector<int> inv;         // reference tet -> internal
+    std::vector<int> tet_cube;    // per internal tet
+    std::vector<int> cube_start;  // CSR over cubes, internal tets sorted by cube
+    std::vector<double> tcoord;   // internal tets: [N][4][3]
+    std::vector<double> vol;      // internal tets
+    std::vector<cd> param;        // internal tets
+};
+
+// Builds the grid/cube structures; throws on orientation and grid errors.
+void build_geometry(const scadyn_mesh &mesh, const scadyn_opts &opts, HostGeometry &g);
+
+// Near-zone pair list in the reference's storage order: for block p,
+// (T[p], B[p]) are internal tet ids with reference ids perm[T] <= perm[B],
+// off[p] in [0,27) is the cube offset (dx+1) + 3(dy+1) + 9(dz+1) from T's cube
+// to B's cube.
+struct NearPairs {
+    std::vector<int> T, B;
+    std::vector<uint8_t> off;
+    // row lists over internal tets: entries (column tet, block id | TRANSPOSE)
+    std::vector<int64_t> row_start;
+    std::vector<int> ent_col;
+    std::vector<uint32_t> ent_blk;
+};
+constexpr uint32_t BLK_TRANSPOSE = 0x80000000u;
+void build_near_pairs(const HostGeometry &g, NearPairs &np);
+
+// SVD pseudo-inverse (one-sided Jacobi), singular values <= thr dropped.
+// A is m x n column-major; returns n x m column-major.
+std::vector<cd> pinv_jacobi(const std::vector<cd> &A, int m, int n, double thr);
+
+// Smallest 2,3,5,7-smooth length >= n, and its radix list (largest first).
+int smooth_length(int n);
+std::vector<int> factor_radices(int L);
+
+}  // namespace scadyn

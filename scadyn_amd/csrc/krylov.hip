@@ -1,0 +1,399 @@
+// Batched GMRES kernels and the T-matrix projection GEMM (FP64).
+#include <algorithm>
+
+#include "krylov.hpp"
+
+namespace scadyn {
+
+// ---- reductions over rows of an RHS-innermost multi-vector --------------------
+// A workgroup of 256 threads covers SB = min(B,256) slots x RW = 256/SB rows per
+// iteration; blockIdx.y selects the slot chunk.  Partial sums per workgroup are
+// combined in a fixed order by the *_reduce kernels (deterministic).
+struct RowSlotMap {
+    int SB, RW, sl, rl, b;
+    bool ok;
+};
+__device__ inline RowSlotMap row_slot_map(int B) {
+    RowSlotMap r;
+    r.SB = B < 256 ? B : 256;
+    r.RW = 256 / r.SB;
+    r.sl = threadIdx.x % r.SB;
+    r.rl = threadIdx.x / r.SB;
+    r.b = blockIdx.y * r.SB + r.sl;
+    r.ok = r.rl < r.RW && r.b < B;
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_mgs_pass(KrylovDims d, int k, cd *__restrict__ W,
+                                                  const cd *__restrict__ V, const cd *__restrict__ Rhs,
+                                                  const SlotState *__restrict__ st, const cd *__restrict__ H,
+                                                  cd *__restrict__ pdot, double *__restrict__ pnrm,
+                                                  double *__restrict__ pnav) {
+    __shared__ cd sdot[256];
+    __shared__ double snrm[256], snav[256];
+    RowSlotMap m = row_slot_map(d.B);
+    cd dot = cmk(0, 0);
+    double nrm = 0.0, nav = 0.0;
+    if (m.ok) {
+        const SlotState s = st[m.b];
+        const size_t nB = (size_t)d.n * d.B;
+        int mode = 0;  // 1: arnoldi pass, 2: restart residual
+        if (s.phase == PH_ARNOLDI && k <= s.i + 1) mode = 1;
+        if (s.phase == PH_RESTART && k == 0) mode = 2;
+        if (mode) {
+            cd h = cmk(0, 0);
+            if (mode == 1 && k >= 1) h = H[(size_t)m.b * (d.m + 1) * d.m + (k - 1) + (size_t)(d.m + 1) * s.i];
+            const cd *Vkm1 = V + (size_t)(k >= 1 ? k - 1 : 0) * nB;
+            const cd *Vk = V + (size_t)k * nB;
+            for (long long row = (long long)blockIdx.x * m.RW + m.rl; row < d.n; row += (long long)gridDim.x * m.RW) {
+                const size_t e = (size_t)row * d.B + m.b;
+                cd w = W[e];
+                if (mode == 2) {
+                    w = csub(Rhs[e], w);
+                    W[e] = w;
+                    nrm += cabs2(w);
+                    continue;
+                }
+                if (k == 0) nav += cabs2(w);
+                if (k >= 1) {
+                    // w = w - h(k-1,i) v_{k-1}
+                    const cd v = Vkm1[e];
+                    w = csub(w, cmul(h, v));
+                    W[e] = w;
+                }
+                if (k <= s.i) {
+                    // h(k,i) = dot_product(v_k, w) = sum conj(v_k) w
+                    const cd v = Vk[e];
+                    dot = cadd(dot, cmulc(v, w));
+                }
+                if (k == s.i + 1) nrm += cabs2(w);
+            }
+        }
+    }
+    sdot[threadIdx.x] = dot;
+    snrm[threadIdx.x] = nrm;
+    snav[threadIdx.x] = nav;
+    __syncthreads();
+    if (m.ok && m.rl == 0) {
+        cd a = cmk(0, 0);
+        double b1 = 0, b2 = 0;
+        for (int r = 0; r < m.RW; ++r) {
+            a = cadd(a, sdot[r * m.SB + m.sl]);
+            b1 += snrm[r * m.SB + m.sl];
+            b2 += snav[r * m.SB + m.sl];
+        }
+        const size_t o = (size_t)blockIdx.x * d.B + m.b;
+        pdot[o] = a;
+        pnrm[o] = b1;
+        pnav[o] = b2;
+    }
+}
+
+__global__ void k_mgs_reduce(KrylovDims d, int k, const SlotState *__restrict__ st, cd *__restrict__ H,
+                             const cd *__restrict__ pdot, const double *__restrict__ pnrm,
+                             const double *__restrict__ pnav, double *__restrict__ nrm2,
+                             double *__restrict__ nav2) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= d.B) return;
+    const SlotState s = st[b];
+    const bool arn = s.phase == PH_ARNOLDI && k <= s.i + 1;
+    const bool rst = s.phase == PH_RESTART && k == 0;
+    if (!arn && !rst) return;
+    cd a = cmk(0, 0);
+    double x1 = 0, x2 = 0;
+    for (int w = 0; w < d.nwg; ++w) {
+        const size_t o = (size_t)w * d.B + b;
+        a = cadd(a, pdot[o]);
+        x1 += pnrm[o];
+        x2 += pnav[o];
+    }
+    if (rst) {
+        nrm2[b] = x1;
+        return;
+    }
+    if (k <= s.i) H[(size_t)b * (d.m + 1) * d.m + k + (size_t)(d.m + 1) * s.i] = a;
+    if (k == s.i + 1) nrm2[b] = x1;
+    if (k == 0) nav2[b] = x2;
+}
+
+// per-slot scalar work: Givens rotations, convergence test, least squares
+__global__ void k_slot_update(KrylovDims d, SlotState *__restrict__ st, cd *__restrict__ Hall,
+                              cd *__restrict__ csall, cd *__restrict__ snall, cd *__restrict__ gall,
+                              cd *__restrict__ yall, const double *__restrict__ nrm2,
+                              const double *__restrict__ nav2, double tol, int restart) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= d.B) return;
+    SlotState s = st[b];
+    const int m = d.m, ld = m + 1;
+    cd *H = Hall + (size_t)b * ld * m;
+    cd *cs = csall + (size_t)b * ld;
+    cd *sn = snall + (size_t)b * ld;
+    cd *g = gall + (size_t)b * ld;
+    cd *y = yall + (size_t)b * m;
+    s.act = ACT_NONE;
+    if (s.phase == PH_RESTART) {
+        const double beta = sqrt(nrm2[b]);
+        for (int q = 0; q <= m; ++q) g[q] = cmk(0, 0);
+        g[0] = cmk(beta, 0);
+        s.beta = beta;
+        if (beta == 0.0) {  // x is exact; the reference would divide by zero here
+            s.phase = PH_DONE;
+            s.conv = 1;
+        } else {
+            s.phase = PH_ARNOLDI;
+            s.i = 0;
+            s.act = ACT_V0;
+            s.tgt = 0;
+        }
+    } else if (s.phase == PH_ARNOLDI) {
+        const int i = s.i;
+        const double hn = sqrt(nrm2[b]);
+        const double nav = sqrt(nav2[b]);
+        H[(i + 1) + (size_t)ld * i] = cmk(hn, 0);
+        if (nav + 0.001 * hn == nav) s.reorth++;
+        s.hnorm = hn;
+        for (int kk = 0; kk < i; ++kk) {
+            cd hk = H[kk + (size_t)ld * i], hk1 = H[kk + 1 + (size_t)ld * i];
+            cd tmp = csub(cmul(cs[kk], hk), cmul(sn[kk], hk1));
+            H[kk + 1 + (size_t)ld * i] = cadd(cmul(sn[kk], hk), cmul(cconj(cs[kk]), hk1));
+            H[kk + (size_t)ld * i] = tmp;
+        }
+        cd hii = H[i + (size_t)ld * i], hi1 = H[i + 1 + (size_t)ld * i];
+        const double nu = sqrt(cabs2(hii) + cabs2(hi1));
+        if (nu != 0.0) {
+            cs[i] = cconj(cmk(hii.x / nu, hii.y / nu));
+            sn[i] = cmk(-hi1.x / nu, -hi1.y / nu);
+            H[i + (size_t)ld * i] = csub(cmul(cs[i], hii), cmul(sn[i], hi1));
+            H[i + 1 + (size_t)ld * i] = cmk(0, 0);
+            cd t1 = g[i], t2 = g[i + 1];
+            g[i] = csub(cmul(cs[i], t1), cmul(sn[i], t2));
+            g[i + 1] = cadd(cmul(sn[i], t1), cmul(cconj(cs[i]), t2));
+        }
+        s.err = sqrt(cabs2(g[i + 1])) / s.bnorm;
+        int len = 0;
+        if (s.err < tol) {
+            len = i + 1;
+            s.conv = 1;
+            s.phase = PH_DONE;
+        } else {
+            s.ite++;
+            if (i + 1 == m) {
+                len = m;
+                s.cycle++;
+                s.phase = (s.cycle >= restart) ? PH_DONE : PH_RESTART;
+            } else {
+                s.act = ACT_NEXT;
+                s.tgt = i + 1;
+                s.i = i + 1;
+            }
+        }
+        if (len > 0) {
+            // back substitution on the rotated (upper triangular) H(1:len,1:len)
+            bool sing = false;
+            for (int j = len - 1; j >= 0; --j) {
+                cd acc = g[j];
+                for (int l = j + 1; l < len; ++l) acc = csub(acc, cmul(H[j + (size_t)ld * l], y[l]));
+                cd djj = H[j + (size_t)ld * j];
+                if (djj.x == 0.0 && djj.y == 0.0) sing = true;
+                y[j] = cdiv(acc, djj);
+            }
+            if (sing) {
+                s.conv = -1;
+                s.phase = PH_DONE;
+            } else {
+                s.act = ACT_XUPD;
+                s.len = len;
+            }
+        }
+    }
+    st[b] = s;
+}
+
+__global__ __launch_bounds__(256) void k_vec_update(KrylovDims d, const SlotState *__restrict__ st,
+                                                    const cd *__restrict__ W, cd *__restrict__ V,
+                                                    const cd *__restrict__ yall) {
+    const size_t nB = (size_t)d.n * d.B;
+    const long long tot = d.n * d.B;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int b = (int)(e % d.B);
+        const SlotState &s = st[b];
+        if (s.act == ACT_V0 || s.act == ACT_NEXT) {
+            const double h = (s.act == ACT_V0) ? s.beta : s.hnorm;
+            cd w = W[e];
+            if (h != 0.0) w = cmk(w.x / h, w.y / h);
+            V[(size_t)s.tgt * nB + e] = w;
+        } else if (s.act == ACT_XUPD) {
+            const cd *y = yall + (size_t)b * d.m;
+            cd acc = cmk(0, 0);
+            for (int q = 0; q < s.len; ++q) cfma(acc, V[(size_t)q * nB + e], y[q]);
+            cd *x = V + (size_t)(d.m + 1) * nB + e;
+            *x = cadd(*x, acc);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_load_rhs(KrylovDims d, const int *__restrict__ col,
+                                                  const cd *__restrict__ src, long long ld,
+                                                  cd *__restrict__ Rhs, cd *__restrict__ V,
+                                                  double *__restrict__ pnrm) {
+    __shared__ double snrm[256];
+    RowSlotMap m = row_slot_map(d.B);
+    double nrm = 0.0;
+    if (m.ok && col[m.b] >= 0) {
+        const size_t nB = (size_t)d.n * d.B;
+        const int c = col[m.b];
+        for (long long row = (long long)blockIdx.x * m.RW + m.rl; row < d.n; row += (long long)gridDim.x * m.RW) {
+            const size_t e = (size_t)row * d.B + m.b;
+            cd v = src[(size_t)row * ld + c];
+            Rhs[e] = v;
+            V[(size_t)(d.m + 1) * nB + e] = v;  // x0 = rhs (src/gmres_module.f90:198)
+            nrm += cabs2(v);
+        }
+    }
+    snrm[threadIdx.x] = nrm;
+    __syncthreads();
+    if (m.ok && m.rl == 0) {
+        double a = 0;
+        for (int r = 0; r < m.RW; ++r) a += snrm[r * m.SB + m.sl];
+        pnrm[(size_t)blockIdx.x * d.B + m.b] = a;
+    }
+}
+
+__global__ void k_load_finish(KrylovDims d, const int *__restrict__ col, SlotState *__restrict__ st,
+                              const double *__restrict__ pnrm) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= d.B || col[b] < 0) return;
+    double a = 0;
+    for (int w = 0; w < d.nwg; ++w) a += pnrm[(size_t)w * d.B + b];
+    SlotState s;
+    s.phase = PH_RESTART;
+    s.i = 0;
+    s.cycle = 0;
+    s.ite = 0;
+    s.conv = 0;
+    s.act = ACT_NONE;
+    s.tgt = 0;
+    s.len = 0;
+    s.reorth = 0;
+    s.col = col[b];
+    s.pad[0] = s.pad[1] = 0;
+    s.bnorm = sqrt(a);
+    s.beta = 0;
+    s.hnorm = 0;
+    s.err = 0;
+    if (s.bnorm == 0.0) {  // zero right-hand side: x = 0
+        s.phase = PH_DONE;
+        s.conv = 1;
+    }
+    st[b] = s;
+}
+
+__global__ void k_store_x(KrylovDims d, const int *__restrict__ col, const cd *__restrict__ V,
+                          cd *__restrict__ dst, long long ld) {
+    const size_t nB = (size_t)d.n * d.B;
+    const long long tot = d.n * d.B;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int b = (int)(e % d.B);
+        const int c = col[b];
+        if (c < 0) continue;
+        const long long row = e / d.B;
+        dst[(size_t)row * ld + c] = V[(size_t)(d.m + 1) * nB + e];
+    }
+}
+
+// ---- T = mat^H X, split over row chunks -----------------------------------------
+__global__ __launch_bounds__(256) void k_tgemm(const cd *__restrict__ A, const cd *__restrict__ X, long long n,
+                                               int ncol, long long chunk, cd *__restrict__ part) {
+    __shared__ cd sa[32][17], sx[32][17];
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    const int i = blockIdx.x * 16 + ty, j = blockIdx.y * 16 + tx;
+    const long long r0 = blockIdx.z * chunk, r1 = std::min(n, r0 + chunk);
+    cd acc = cmk(0, 0);
+    for (long long rb = r0; rb < r1; rb += 32) {
+        for (int e = threadIdx.x; e < 32 * 16; e += 256) {
+            const int rr = e / 16, cc = e % 16;
+            const long long row = rb + rr;
+            const int ia = blockIdx.x * 16 + cc, jx = blockIdx.y * 16 + cc;
+            sa[rr][cc] = (row < r1 && ia < ncol) ? A[(size_t)row * ncol + ia] : cmk(0, 0);
+            sx[rr][cc] = (row < r1 && jx < ncol) ? X[(size_t)row * ncol + jx] : cmk(0, 0);
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int rr = 0; rr < 32; ++rr) acc = cadd(acc, cmulc(sa[rr][ty], sx[rr][tx]));
+        __syncthreads();
+    }
+    if (i < ncol && j < ncol) part[((size_t)blockIdx.z * ncol + i) * ncol + j] = acc;
+}
+
+__global__ void k_tgemm_reduce(const cd *__restrict__ part, int nz, int ncol, cd *__restrict__ T) {
+    const size_t tot = (size_t)ncol * ncol;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
+        cd a = cmk(0, 0);
+        for (int z = 0; z < nz; ++z) a = cadd(a, part[(size_t)z * tot + e]);
+        T[e] = a;
+    }
+}
+
+// ---- launchers -------------------------------------------------------------------------
+static dim3 rs_grid(const KrylovDims &d) {
+    int SB = std::min(d.B, 256);
+    return dim3((unsigned)d.nwg, (unsigned)((d.B + SB - 1) / SB));
+}
+
+void launch_mgs_pass(const KrylovDims &d, int k, cd *W, const cd *V, const cd *Rhs, const SlotState *st,
+                     const cd *H, cd *pdot, double *pnrm, double *pnav, hipStream_t s) {
+    hipLaunchKernelGGL(k_mgs_pass, rs_grid(d), dim3(256), 0, s, d, k, W, V, Rhs, st, H, pdot, pnrm, pnav);
+    HIP_CHECK(hipGetLastError());
+}
+void launch_mgs_reduce(const KrylovDims &d, int k, const SlotState *st, cd *H, const cd *pdot,
+                       const double *pnrm, const double *pnav, double *nrm2, double *nav2, hipStream_t s) {
+    hipLaunchKernelGGL(k_mgs_reduce, dim3((d.B + 63) / 64), dim3(64), 0, s, d, k, st, H, pdot, pnrm, pnav, nrm2,
+                       nav2);
+    HIP_CHECK(hipGetLastError());
+}
+void launch_slot_update(const KrylovDims &d, SlotState *st, cd *H, cd *cs, cd *sn, cd *g, cd *y,
+                        const double *nrm2, const double *nav2, double tol, int restart, hipStream_t s) {
+    hipLaunchKernelGGL(k_slot_update, dim3((d.B + 63) / 64), dim3(64), 0, s, d, st, H, cs, sn, g, y, nrm2, nav2,
+                       tol, restart);
+    HIP_CHECK(hipGetLastError());
+}
+void launch_vec_update(const KrylovDims &d, const SlotState *st, const cd *W, cd *V, const cd *y,
+                       hipStream_t s) {
+    long long tot = d.n * d.B;
+    unsigned g = (unsigned)std::min<long long>((tot + 255) / 256, 1 << 16);
+    hipLaunchKernelGGL(k_vec_update, dim3(g), dim3(256), 0, s, d, st, W, V, y);
+    HIP_CHECK(hipGetLastError());
+}
+void launch_load_rhs(const KrylovDims &d, const int *col, const cd *src, long long ld, cd *Rhs, cd *V,
+                     double *pnrm, hipStream_t s) {
+    hipLaunchKernelGGL(k_load_rhs, rs_grid(d), dim3(256), 0, s, d, col, src, ld, Rhs, V, pnrm);
+    HIP_CHECK(hipGetLastError());
+}
+void launch_load_finish(const KrylovDims &d, const int *col, SlotState *st, const double *pnrm,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_load_finish, dim3((d.B + 63) / 64), dim3(64), 0, s, d, col, st, pnrm);
+    HIP_CHECK(hipGetLastError());
+}
+void launch_store_x(const KrylovDims &d, const int *col, const cd *V, cd *dst, long long ld, hipStream_t s) {
+    long long tot = d.n * d.B;
+    unsigned g = (unsigned)std::min<long long>((tot + 255) / 256, 1 << 16);
+    hipLaunchKernelGGL(k_store_x, dim3(g), dim3(256), 0, s, d, col, V, dst, ld);
+    HIP_CHECK(hipGetLastError());
+}
+void launch_tgemm(const cd *mat, const cd *X, long long n, int ncol, cd *T, cd *work, size_t work_elems,
+                  hipStream_t s) {
+    const int tiles = (ncol + 15) / 16;
+    long long nz = std::max<long long>(1, std::min<long long>((long long)(work_elems / ((size_t)ncol * ncol)),
+                                                               std::max<long long>(1, 4096 / (tiles * tiles))));
+    nz = std::min<long long>(nz, std::max<long long>(1, n / 64));
+    const long long chunk = (n + nz - 1) / nz;
+    nz = (n + chunk - 1) / chunk;
+    hipLaunchKernelGGL(k_tgemm, dim3(tiles, tiles, (unsigned)nz), dim3(256), 0, s, mat, X, n, ncol, chunk, work);
+    HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_tgemm_reduce, dim3(256), dim3(256), 0, s, work, (int)nz, ncol, T);
+    HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace scadyn

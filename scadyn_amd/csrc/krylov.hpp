@@ -1,0 +1,58 @@
+// Batched restarted GMRES (src/gmres_module.f90:180-325) on RHS-innermost
+// multi-vectors: every slot b runs its own GMRES(m) with the reference's
+// semantics (x0 = rhs, modified Gram-Schmidt, complex Givens rotations,
+// stopping on |g(i+1)|/||b|| < tol, `restart` cycles); slots advance one
+// matvec per step and are refilled independently.
+#pragma once
+
+#include "internal.hpp"
+
+namespace scadyn {
+
+enum SlotPhase : int { PH_IDLE = 0, PH_RESTART = 1, PH_ARNOLDI = 2, PH_DONE = 3 };
+enum SlotAction : int { ACT_NONE = 0, ACT_V0 = 1, ACT_NEXT = 2, ACT_XUPD = 3 };
+
+struct SlotState {
+    int phase;
+    int i;        // current Arnoldi column (0-based) while PH_ARNOLDI
+    int cycle;    // restart cycle
+    int ite;      // the reference's iteration counter
+    int conv;     // 1 when |g(i+1)|/||b|| < tol was reached
+    int act;      // SlotAction for the vector update
+    int tgt;      // Krylov index written by ACT_V0 / ACT_NEXT
+    int len;      // number of y entries for ACT_XUPD
+    int reorth;   // times the reference's re-orthogonalisation test fired
+    int col;      // RHS column held by the slot (-1 none)
+    int pad[2];
+    double bnorm, beta, hnorm, err;
+};
+
+struct KrylovDims {
+    long long n;  // rows (3N)
+    int B;        // slots
+    int m;        // Krylov dimension (mesh%maxit)
+    int nwg;      // workgroups of the reduction kernels
+};
+
+// V holds m+2 blocks of n*B: Krylov vectors 0..m and X at index m+1.
+void launch_mgs_pass(const KrylovDims &d, int k, cd *W, const cd *V, const cd *Rhs, const SlotState *st,
+                     const cd *H, cd *pdot, double *pnrm, double *pnav, hipStream_t s);
+void launch_mgs_reduce(const KrylovDims &d, int k, const SlotState *st, cd *H, const cd *pdot,
+                       const double *pnrm, const double *pnav, double *nrm2, double *nav2, hipStream_t s);
+void launch_slot_update(const KrylovDims &d, SlotState *st, cd *H, cd *cs, cd *sn, cd *g, cd *y,
+                        const double *nrm2, const double *nav2, double tol, int restart, hipStream_t s);
+void launch_vec_update(const KrylovDims &d, const SlotState *st, const cd *W, cd *V, const cd *y,
+                       hipStream_t s);
+// load column col[b] (>= 0) of src (n x ld, row-major) into Rhs and X of slot b
+void launch_load_rhs(const KrylovDims &d, const int *col, const cd *src, long long ld, cd *Rhs, cd *V,
+                     double *pnrm, hipStream_t s);
+void launch_load_finish(const KrylovDims &d, const int *col, SlotState *st, const double *pnrm,
+                        hipStream_t s);
+// store X of slots with col[b] >= 0 into column col[b] of dst (n x ld)
+void launch_store_x(const KrylovDims &d, const int *col, const cd *V, cd *dst, long long ld, hipStream_t s);
+
+// T = mat^H X: mat, X are n x ncol row-major; T is ncol x ncol row-major (i, j)
+void launch_tgemm(const cd *mat, const cd *X, long long n, int ncol, cd *T, cd *work, size_t work_elems,
+                  hipStream_t s);
+
+}  // namespace scadyn

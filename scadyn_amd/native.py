@@ -1,0 +1,223 @@
+"""ctypes binding of libscadyn_amd.so (the C ABI in include/scadyn_amd.h).
+
+The library is built in-tree by scadyn_amd/build.py.  There is no fallback:
+if the shared library is missing or fails to load, every entry point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libscadyn_amd.so')
+
+OK = 0
+ERR_ARG, ERR_ORIENTATION, ERR_GRID, ERR_SINGULAR, ERR_NOCONV, ERR_MEMORY, ERR_DEVICE, ERR_UNSUPPORTED = range(1, 9)
+PREC_FP64, PREC_MIXED = 0, 1
+EXPORT_S, EXPORT_BLOCKS, EXPORT_BLKIND, EXPORT_FG = 1, 2, 3, 4
+KCLASS = ['scatter', 'fftx', 'ffty', 'fftz_conv', 'iffty', 'ifftx', 'gather_near', 'krylov']
+
+# every symbol include/scadyn_amd.h declares
+EXPORTED = ['scadyn_opts_default', 'scadyn_last_error', 'scadyn_tmat_solve', 'scadyn_op_create',
+            'scadyn_op_destroy', 'scadyn_op_info', 'scadyn_op_export', 'scadyn_op_matvec',
+            'scadyn_op_solve', 'scadyn_op_vswf', 'scadyn_op_tmatrix', 'scadyn_op_profile',
+            'scadyn_op_timings']
+
+
+class ScadynError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f'scadyn error {code}: {msg}')
+        self.code = code
+
+
+class CMesh(ctypes.Structure):
+    _fields_ = [('n_node', ctypes.c_int32), ('n_tet', ctypes.c_int32),
+                ('coord', ctypes.POINTER(ctypes.c_double)),
+                ('etopol', ctypes.POINTER(ctypes.c_int32)),
+                ('param', ctypes.POINTER(ctypes.c_double))]
+
+
+class COpts(ctypes.Structure):
+    _fields_ = [('tol', ctypes.c_double), ('maxit', ctypes.c_int32), ('restart', ctypes.c_int32),
+                ('M_ex', ctypes.c_int32), ('near_zone', ctypes.c_int32), ('grid_size', ctypes.c_double),
+                ('order', ctypes.c_int32), ('precision', ctypes.c_int32), ('batch', ctypes.c_int32),
+                ('strict', ctypes.c_int32), ('device', ctypes.c_int32), ('fft_pad', ctypes.c_int32),
+                ('refine_tol', ctypes.c_double), ('reserved', ctypes.c_int32 * 8)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f'{LIB_PATH} is not built; run scadyn_amd/build.py (or __graft_entry__.build())')
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int32)
+    L.scadyn_opts_default.argtypes = [ctypes.POINTER(COpts)]
+    L.scadyn_opts_default.restype = None
+    L.scadyn_last_error.argtypes = []
+    L.scadyn_last_error.restype = ctypes.c_char_p
+    L.scadyn_tmat_solve.argtypes = [ctypes.POINTER(CMesh), ctypes.POINTER(COpts), ctypes.c_double,
+                                    ctypes.c_int32, dp, dp, dp, dp, ip]
+    L.scadyn_op_create.argtypes = [ctypes.POINTER(CMesh), ctypes.POINTER(COpts), ctypes.c_double,
+                                   ctypes.POINTER(P)]
+    L.scadyn_op_destroy.argtypes = [P]
+    L.scadyn_op_destroy.restype = None
+    L.scadyn_op_info.argtypes = [P, ctypes.POINTER(ctypes.c_int64), dp]
+    L.scadyn_op_export.argtypes = [P, ctypes.c_int32, P, ctypes.c_int64]
+    L.scadyn_op_matvec.argtypes = [P, ctypes.c_int32, dp, dp]
+    L.scadyn_op_solve.argtypes = [P, ctypes.c_int32, dp, dp, ip]
+    L.scadyn_op_vswf.argtypes = [P, ctypes.c_int32, dp]
+    L.scadyn_op_tmatrix.argtypes = [P, ctypes.c_int32, dp, dp, dp, dp, ip]
+    L.scadyn_op_profile.argtypes = [P, ctypes.c_int32]
+    L.scadyn_op_timings.argtypes = [P, dp]
+    for name in EXPORTED:
+        if name not in ('scadyn_opts_default', 'scadyn_last_error', 'scadyn_op_destroy'):
+            getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != OK:
+        raise ScadynError(rc, lib().scadyn_last_error().decode())
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def default_opts(**kw):
+    o = COpts()
+    lib().scadyn_opts_default(ctypes.byref(o))
+    for key, val in kw.items():
+        setattr(o, key, val)
+    return o
+
+
+class MeshArrays:
+    """Keeps the contiguous arrays alive while the C mesh struct points at them.
+
+    coord (Nnode,3) metres, etopol (N,4) 0-based, param (N,) complex.
+    """
+
+    def __init__(self, coord, etopol, param):
+        self.coord = np.ascontiguousarray(coord, dtype=np.float64)
+        self.etopol = np.ascontiguousarray(np.asarray(etopol) + 1, dtype=np.int32)
+        self.param = np.ascontiguousarray(np.asarray(param, dtype=np.complex128))
+        self.c = CMesh(self.coord.shape[0], self.etopol.shape[0], _dp(self.coord), _ip(self.etopol),
+                       self.param.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+
+
+class Operator:
+    """Device-resident pFFT operator for one mesh and wavenumber."""
+
+    def __init__(self, coord, etopol, param, k, **opts):
+        self.mesh = MeshArrays(coord, etopol, param)
+        self.opts = default_opts(**opts)
+        self.k = float(k)
+        h = ctypes.c_void_p()
+        check(lib().scadyn_op_create(ctypes.byref(self.mesh.c), ctypes.byref(self.opts), self.k,
+                                     ctypes.byref(h)))
+        self.h = h
+        self.N = self.mesh.etopol.shape[0]
+
+    def close(self):
+        if getattr(self, 'h', None):
+            lib().scadyn_op_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        i = np.zeros(16, np.int64)
+        d = np.zeros(4)
+        check(lib().scadyn_op_info(self.h, i.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), _dp(d)))
+        keys = ['N_tet', 'Nx', 'Ny', 'Nz', 'N_cubes', 'N_tet_cube', 'n_blocks', 'Lx', 'Ly', 'Lz',
+                'batch', 'device_bytes']
+        out = {k: int(i[j]) for j, k in enumerate(keys)}
+        out['delta'], out['box_delta'] = float(d[0]), float(d[1])
+        return out
+
+    def export(self, what):
+        inf = self.info()
+        N = self.N
+        if what == 'S':
+            a = np.zeros((4, N, 27), np.complex128)
+            wid = EXPORT_S
+        elif what == 'blocks':
+            a = np.zeros((inf['n_blocks'], 3, 3), np.complex64)  # [p][j][i] col-major -> transpose below
+            wid = EXPORT_BLOCKS
+        elif what == 'blkind':
+            a = np.zeros((inf['n_blocks'], 2), np.int32)
+            wid = EXPORT_BLKIND
+        elif what == 'fg':
+            a = np.zeros((inf['Lz'] // 2 + 1, inf['Ly'] // 2 + 1, inf['Lx'] // 2 + 1), np.complex128)
+            wid = EXPORT_FG
+        else:
+            raise ValueError(what)
+        check(lib().scadyn_op_export(self.h, wid, a.ctypes.data_as(ctypes.c_void_p), a.nbytes))
+        if what == 'blocks':
+            a = np.ascontiguousarray(a.transpose(0, 2, 1))  # -> [p][i][j]
+        return a
+
+    def matvec(self, x):
+        x = np.ascontiguousarray(np.atleast_2d(x), dtype=np.complex128)
+        y = np.zeros_like(x)
+        check(lib().scadyn_op_matvec(self.h, x.shape[0], _dp(x.view(np.float64)), _dp(y.view(np.float64))))
+        return y
+
+    def solve(self, rhs):
+        rhs = np.ascontiguousarray(np.atleast_2d(rhs), dtype=np.complex128)
+        x = np.zeros_like(rhs)
+        it = np.zeros(rhs.shape[0], np.int32)
+        check(lib().scadyn_op_solve(self.h, rhs.shape[0], _dp(rhs.view(np.float64)),
+                                    _dp(x.view(np.float64)), _ip(it)))
+        return x, it
+
+    def vswf(self, nmax):
+        nm = (nmax + 1) ** 2 - 1
+        mat = np.zeros((2 * nm, 3 * self.N), np.complex128)  # column-major (3N, 2nm)
+        check(lib().scadyn_op_vswf(self.h, nmax, _dp(mat.view(np.float64))))
+        return mat.T
+
+    def tmatrix(self, nmax):
+        nm = (nmax + 1) ** 2 - 1
+        T = [np.zeros((nm, nm), np.complex128) for _ in range(4)]  # filled column-major
+        it = np.zeros(2 * nm, np.int32)
+        check(lib().scadyn_op_tmatrix(self.h, nmax, *[_dp(t.view(np.float64)) for t in T], _ip(it)))
+        return tuple(t.T.copy() for t in T), it
+
+    def profile(self, on=True):
+        check(lib().scadyn_op_profile(self.h, 1 if on else 0))
+
+    def timings(self):
+        t = np.zeros((len(KCLASS), 3))
+        check(lib().scadyn_op_timings(self.h, _dp(t)))
+        return {name: {'ms': t[i, 0], 'launches': int(t[i, 1]), 'bytes': t[i, 2]}
+                for i, name in enumerate(KCLASS)}
+
+
+def tmat_solve(coord, etopol, param, k, nmax, **opts):
+    """One wavelength bar through the single-call boundary (scadyn_tmat_solve)."""
+    m = MeshArrays(coord, etopol, param)
+    o = default_opts(**opts)
+    nm = (nmax + 1) ** 2 - 1
+    T = [np.zeros((nm, nm), np.complex128) for _ in range(4)]
+    it = np.zeros(2 * nm, np.int32)
+    check(lib().scadyn_tmat_solve(ctypes.byref(m.c), ctypes.byref(o), float(k), nmax,
+                                  *[_dp(t.view(np.float64)) for t in T], _ip(it)))
+    return tuple(t.T.copy() for t in T), it

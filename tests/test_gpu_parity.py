@@ -1,0 +1,120 @@
+"""HIP path vs the oracle on a seeded 192-tet sphere (ka = 2, eps = 2.5+0.01i).
+
+Tolerances (error budget: the 98x27 pseudo-inverse W+ of the projection has
+cond(W) ~ 1.5e6, so two correct SVD codes give S* that differ by a few
+cond*eps ~ 1e-9; everything downstream inherits that):
+  S (FP64)                              5e-9 relative
+  Fg (FP64 FFT)                         1e-12 relative
+  near-zone blocks (complex64 storage)  2 ulp of complex64 per block norm
+  matvec                                1e-8 relative
+  solve (same iteration counts)         1e-8 relative
+  T-matrix                              1e-7 relative (the parity bar is 1e-6)
+"""
+import numpy as np
+import pytest
+
+from helpers import oracle_operator, rel, sphere_case
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f'[gpu-parity] {msg}', flush=True)
+
+
+@pytest.fixture(scope='module')
+def ops():
+    from scadyn_amd import native
+    c, e, param, k = sphere_case()
+    say('create operator (192 tets, fft_pad=1)')
+    g = native.Operator(c, e, param, k, fft_pad=1)
+    o = oracle_operator()
+    yield g, o
+    g.close()
+
+
+def test_geometry(ops):
+    g, o = ops
+    inf = g.info()
+    assert (inf['Nx'], inf['Ny'], inf['Nz']) == (o.g.Nx, o.g.Ny, o.g.Nz)
+    assert inf['N_cubes'] == o.g.N_cubes
+    assert inf['N_tet_cube'] == o.g.N_tet_cube
+    assert inf['n_blocks'] == o.sp_mat.shape[0]
+    assert abs(inf['delta'] - o.g.delta) <= 1e-15 * o.g.delta
+
+
+def test_projection_S(ops):
+    g, o = ops
+    say('export S')
+    S = g.export('S')
+    for ch, ref in enumerate((o.S, o.Sx, o.Sy, o.Sz)):
+        assert rel(S[ch], ref) < 5e-9, ch
+
+
+def test_green_fft(ops):
+    g, o = ops
+    say('export Fg')
+    fg = g.export('fg')
+    Nx, Ny, Nz = o.g.Nx, o.g.Ny, o.g.Nz
+    ref = o.Fg[:Nz + 1, :Ny + 1, :Nx + 1] / (8.0 * Nx * Ny * Nz)
+    assert rel(fg, ref) < 1e-12
+
+
+def test_near_blocks(ops):
+    g, o = ops
+    say('export blocks')
+    ind = g.export('blkind')
+    assert np.array_equal(ind - 1, o.sp_ind)
+    blk = g.export('blocks').astype(np.complex128)
+    ref = o.sp_mat.astype(np.complex128)
+    scale = np.abs(ref).reshape(len(ref), -1).max(axis=1)
+    err = np.abs(blk - ref).reshape(len(ref), -1).max(axis=1) / scale
+    assert err.max() < 2 * 2.0 ** -23, err.max()
+
+
+def test_matvec(ops):
+    from oracle import pfft_ref
+    g, o = ops
+    rng = np.random.default_rng(1)
+    n = 3 * o.g.N
+    x = rng.standard_normal((3, n)) + 1j * rng.standard_normal((3, n))
+    say('matvec x3')
+    y = g.matvec(x)
+    for r in range(3):
+        assert rel(y[r], pfft_ref.matvec(o, x[r])) < 1e-8
+
+
+def test_solve(ops):
+    from oracle import pfft_ref
+    g, o = ops
+    rng = np.random.default_rng(2)
+    n = 3 * o.g.N
+    b = rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n))
+    say('solve x2')
+    x, it = g.solve(b)
+    for r in range(2):
+        xr, ite, _ = pfft_ref.gmres(lambda v: pfft_ref.matvec(o, v), b[r].copy())
+        assert it[r] == ite
+        assert rel(x[r], xr) < 1e-8
+
+
+def test_vswf(ops):
+    from oracle import pfft_ref
+    g, o = ops
+    c, e, param, k = sphere_case()
+    say('vswf nmax=3')
+    mat = g.vswf(3)
+    ref = pfft_ref.vswf_matrix(c, e, k, 3)
+    assert rel(mat, ref) < 1e-12
+
+
+def test_tmatrix(ops):
+    from oracle import pfft_ref
+    g, o = ops
+    c, e, param, k = sphere_case()
+    say('tmatrix nmax=2')
+    T, it = g.tmatrix(2)
+    Tref, itref = pfft_ref.tmatrix(o, 2, c, e)
+    for a, b in zip(T, Tref):
+        assert rel(a, b) < 1e-7
+    assert list(it) == list(itref)
