@@ -1,0 +1,165 @@
+"""Benchmark: T-matrix columns/s of the JVIE solve on one MI355X.
+
+Workload (the headline configuration): Gaussian random ellipsoid, semi-axes
+1 : 0.8 : 0.6, log-normal radial perturbation sigma = 0.1, correlation angle
+20 deg, real spherical harmonics l <= 10 (seed 0), cube-split to 198,390 tets,
+eps = 3.0 + 0.1i, size parameter ka = 8 (a = volume-equivalent radius),
+nmax = 12, i.e. 336 right-hand sides solved as one batched launch, FP64.
+
+A step is one full T-matrix solve (compute_T_matrix: VSWF right-hand sides,
+336 restarted-GMRES systems, projection GEMM) on the device-resident operator;
+the operator setup (projections, FFT(G), near zone) is built before the timed
+region and reported separately.
+
+The CPU baseline times the oracle (numpy restatement, one core) on a bounded
+sample: its matvec on the full-size operator, scaled by the matvecs per column
+that the GPU solve needed.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = 'T-matrix columns/s, mixed vs FP64 pFFT-GMRES at 1e-6 rel err; p99 latency'
+HBM_PEAK_GBS = 8000.0
+
+
+def make_workload(name):
+    from scadyn_amd import meshgen
+    a = 1e-7
+    if name == 'ellipsoid200k':
+        c, e = meshgen.gaussian_ellipsoid(n_cells=56, seed=0)
+        eps, ka, nmax = 3.0 + 0.1j, 8.0, 12
+        desc = ('Gaussian random ellipsoid 1:0.8:0.6, sigma 0.1, corr 20 deg, l<=10, seed 0; '
+                'cube-split; eps 3.0+0.1i; ka 8; nmax 12; FP64')
+    elif name == 'sphere43k':
+        c, e = meshgen.sphere(24, seed=0)
+        eps, ka, nmax = 2.5 + 0.01j, 2.0, 5
+        desc = 'sphere, 24^3 cells (centre-in), jitter 0.1h seed 0; eps 2.5+0.01i; ka 2; nmax 5; FP64'
+    else:
+        raise SystemExit(f'unknown workload {name}')
+    c = meshgen.rescale_to_radius(c, e, a)
+    param = np.full(e.shape[0], eps)
+    return c, e, param, ka / a, nmax, desc
+
+
+def cpu_baseline(op_gpu, c, e, param, k, matvecs_per_col, budget_s=25.0):
+    """Oracle matvec on the full-size operator (operator arrays exported from the
+    device build), scaled to columns/s with the measured matvecs per column."""
+    from oracle import pfft_ref
+    t0 = time.time()
+    o = pfft_ref.Operator()
+    o.g = pfft_ref.build_geometry(c, e)
+    o.k = k
+    o.param = param
+    S = op_gpu.export('S')
+    o.S, o.Sx, o.Sy, o.Sz = S[0], S[1], S[2], S[3]
+    o.indS = np.stack([pfft_ref.box_nodes(o.g, cc)[0] for cc in o.g.tet_cube])
+    o.Fg = np.fft.fftn(pfft_ref.green_grid(o.g, k))
+    o.sp_mat = op_gpu.export('blocks')
+    o.sp_ind = op_gpu.export('blkind').astype(np.int64) - 1
+    prep = time.time() - t0
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(3 * o.g.N) + 1j * rng.standard_normal(3 * o.g.N)
+    n = 0
+    t0 = time.time()
+    while True:
+        x = pfft_ref.matvec(o, x)
+        x /= np.linalg.norm(x)
+        n += 1
+        if time.time() - t0 > budget_s or n >= 4:
+            break
+    per_mv = (time.time() - t0) / n
+    cols = 1.0 / (per_mv * matvecs_per_col)
+    return {'value': cols, 'unit': 'columns/s', 'cores': 1, 'kind': 'port',
+            'sample': (f'{n} oracle matvecs (numpy, 1 thread) on the full {o.g.N}-tet operator, '
+                       f'{per_mv:.2f} s each, x {matvecs_per_col:.1f} matvecs/column measured on the '
+                       f'GPU; operator arrays exported from the device build ({prep:.0f} s prep)')}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=2)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--workload', default='ellipsoid200k')
+    ap.add_argument('--batch', type=int, default=0)
+    ap.add_argument('--no-cpu', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f'bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU path is not built.',
+              file=sys.stderr)
+        sys.exit(2)
+    from scadyn_amd import native
+    c, e, param, k, nmax, desc = make_workload(args.workload)
+    print(f'[bench] workload {args.workload}: {e.shape[0]} tets, nmax {nmax}', file=sys.stderr, flush=True)
+    t0 = time.time()
+    op = native.Operator(c, e, param, k, batch=args.batch)
+    setup_s = time.time() - t0
+    info = op.info()
+    print(f'[bench] setup {setup_s:.2f} s: {info}', file=sys.stderr, flush=True)
+    ncol = 2 * ((nmax + 1) ** 2 - 1)
+    for w in range(args.warmup):
+        t = time.time()
+        _, it = op.tmatrix(nmax)
+        print(f'[bench] warmup {w}: {time.time() - t:.2f} s, mean iters {it.mean():.1f}', file=sys.stderr,
+              flush=True)
+    op.profile(True)
+    t0 = time.time()
+    iters = []
+    for s in range(args.steps):
+        T, it = op.tmatrix(nmax)
+        iters.append(it)
+        print(f'[bench] step {s}: {time.time() - t0:.2f} s cumulative', file=sys.stderr, flush=True)
+    elapsed = time.time() - t0
+    tim = op.timings()
+    op.profile(False)
+    ms_per_step = 1000.0 * elapsed / args.steps
+    value = ncol * args.steps / elapsed
+    it = np.concatenate(iters)
+    # matvecs per column: `ite` counts unconverged inner iterations; add the
+    # converged one and one restart matvec per cycle
+    mv_per_col = float(np.mean(it + 1 + (it // op.opts.maxit + 1)))
+    mv_classes = ['scatter', 'fftx', 'ffty', 'fftz_conv', 'iffty', 'ifftx', 'gather_near']
+    dom = max(mv_classes, key=lambda n: tim[n]['ms'])
+    d = tim[dom]
+    avg_ms = d['ms'] / max(d['launches'], 1)
+    bytes_per_launch = d['bytes'] / max(d['launches'], 1)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    per_kernel = {n: {'avg_us': 1000.0 * tim[n]['ms'] / max(tim[n]['launches'], 1),
+                      'launches': tim[n]['launches'],
+                      'GBps': (tim[n]['bytes'] / max(tim[n]['launches'], 1))
+                      / max(1e-12, tim[n]['ms'] / max(tim[n]['launches'], 1) * 1e-3) / 1e9}
+                  for n in mv_classes}
+    per_kernel['krylov'] = {'ms_total': tim['krylov']['ms']}
+    print('[bench] kernels ' + json.dumps(per_kernel), file=sys.stderr, flush=True)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(op, c, e, param, k, mv_per_col)
+        print(f"[bench] cpu baseline {cpu['value']:.4g} columns/s", file=sys.stderr, flush=True)
+    out = {
+        'metric': METRIC, 'value': value, 'unit': 'columns/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': ms_per_step, 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f64', 'data': 'synthetic (seeded mesh generator)',
+        'config': {'workload': desc, 'n_tet': int(e.shape[0]), 'columns': ncol, 'nmax': nmax,
+                   'grid': [info['Nx'], info['Ny'], info['Nz']], 'fft': [info['Lx'], info['Ly'], info['Lz']],
+                   'near_blocks': info['n_blocks'], 'batch': info['batch'], 'setup_s': setup_s,
+                   'mean_gmres_iters': float(it.mean()), 'matvecs_per_column': mv_per_col,
+                   'columns_per_s_with_setup': ncol * args.steps / (elapsed + setup_s * args.steps)},
+        'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
+                     'avg_launch_us': avg_ms * 1000.0, 'bytes_per_launch': bytes_per_launch},
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(out), flush=True)
+    op.close()
+
+
+if __name__ == '__main__':
+    main()

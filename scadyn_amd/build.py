@@ -16,6 +16,31 @@ HIPCC_FLAGS = [
 ]
 
 
+FORTRAN_DIR = os.path.join(HERE, 'fortran')
+FORTRAN_BIN = os.path.join(FORTRAN_DIR, '_bin')
+FORTRAN_EXE = os.path.join(FORTRAN_BIN, 'solve_bar')
+
+
+def build_fortran(verbose=False):
+    """Fortran host driver linked against the library (needs amdflang)."""
+    import shutil
+    fc = shutil.which('amdflang')
+    if fc is None:
+        return None
+    os.makedirs(FORTRAN_BIN, exist_ok=True)
+    cmd = [fc, '-O2', '-module-dir', FORTRAN_BIN, '-o', FORTRAN_EXE + '.tmp',
+           os.path.join(FORTRAN_DIR, 'scadyn_amd_iface.f90'), os.path.join(FORTRAN_DIR, 'solve_bar_main.f90'),
+           '-L' + HERE, '-lscadyn_amd', '-Wl,-rpath,$ORIGIN/../..']
+    if verbose:
+        print(' '.join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError('building the Fortran host driver failed')
+    os.replace(FORTRAN_EXE + '.tmp', FORTRAN_EXE)
+    return FORTRAN_EXE
+
+
 def build(verbose=False):
     hipcc = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
@@ -28,6 +53,7 @@ def build(verbose=False):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError('hipcc failed building libscadyn_amd.so')
     os.replace(tmp, LIB)
+    build_fortran(verbose)
     return LIB
 
 
