@@ -91,6 +91,9 @@ def main():
     ap.add_argument('--workload', default='ellipsoid200k')
     ap.add_argument('--batch', type=int, default=0)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--precision', default='fp64', choices=['fp64', 'mixed'])
+    ap.add_argument('--tol', type=float, default=1e-5)
+    ap.add_argument('--refine-tol', type=float, default=1e-9)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU path is not built.',
@@ -100,7 +103,9 @@ def main():
     c, e, param, k, nmax, desc = make_workload(args.workload)
     print(f'[bench] workload {args.workload}: {e.shape[0]} tets, nmax {nmax}', file=sys.stderr, flush=True)
     t0 = time.time()
-    op = native.Operator(c, e, param, k, batch=args.batch)
+    prec = native.PREC_MIXED if args.precision == 'mixed' else native.PREC_FP64
+    op = native.Operator(c, e, param, k, batch=args.batch, precision=prec, tol=args.tol,
+                         refine_tol=args.refine_tol)
     setup_s = time.time() - t0
     info = op.info()
     print(f'[bench] setup {setup_s:.2f} s: {info}', file=sys.stderr, flush=True)

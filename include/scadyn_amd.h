@@ -121,10 +121,15 @@ int scadyn_op_export(const scadyn_operator *op, int32_t what, void *host, int64_
 
 /* y = A x for nrhs vectors x[nrhs][3N] (compute_Ax2, src/gmres_module.f90:57-176). */
 int scadyn_op_matvec(scadyn_operator *op, int32_t nrhs, const double *x, double *y);
+/* The same with the matvec precision chosen explicitly (SCADYN_PREC_MIXED: S*,
+ * grids and FFTs in complex64; x and y stay complex128). */
+int scadyn_op_matvec_prec(scadyn_operator *op, int32_t precision, int32_t nrhs, const double *x, double *y);
 
 /* Solve A x = rhs for nrhs right-hand sides [nrhs][3N] with restarted GMRES
  * (src/gmres_module.f90:180-325).  iters[nrhs] (optional) receives the
- * reference's iteration count `ite` of each system. */
+ * reference's iteration count `ite` of each system (summed over the inner
+ * solves in mixed precision, which refines with FP64 residuals until
+ * ||b - A x|| <= refine_tol ||b||). */
 int scadyn_op_solve(scadyn_operator *op, int32_t nrhs, const double *rhs, double *x,
                     int32_t *iters);
 

@@ -118,7 +118,7 @@ struct Workspace {
     int B = 0, m = 0;
     long long n = 0;
     DevMem mem;
-    cd *grid0 = nullptr, *grid1 = nullptr;
+    void *grid0 = nullptr, *grid1 = nullptr;  // complex128 or complex64 grids (one matvec at a time)
     cd *V = nullptr, *Rhs = nullptr, *W = nullptr;
     SlotState *st = nullptr;
     cd *H = nullptr, *cs = nullptr, *sn = nullptr, *g = nullptr, *y = nullptr;
@@ -127,7 +127,7 @@ struct Workspace {
     long long *in_off = nullptr;
     int *col = nullptr, *active = nullptr;
     int nwg = 0;
-    LinePass px, py, pz, qy, qx;
+    LinePass pass[2][5];  // [fp64, fp32][x fwd, y fwd, z conv, y inv, x inv]
 };
 
 }  // namespace scadyn
@@ -146,6 +146,7 @@ struct scadyn_operator {
     DevMem mem;
     double *d_tcoord = nullptr, *d_vol = nullptr;
     cd *d_param = nullptr, *d_S4 = nullptr, *d_fg = nullptr, *d_tw = nullptr;
+    cf *d_S4f = nullptr, *d_fgf = nullptr, *d_twf = nullptr;  // complex64 copies (mixed precision)
     int *d_tet_cube = nullptr, *d_cube_start = nullptr, *d_perm = nullptr;
     cf *d_blocks = nullptr;
     int64_t *d_row_start = nullptr;
@@ -154,6 +155,7 @@ struct scadyn_operator {
     long long nblocks = 0;
     std::unique_ptr<Workspace> ws;
     Profiler prof;
+    int32_t last_outer_rounds = 0;
     ~scadyn_operator() {
         ws.reset();
         mem.clear();
@@ -179,13 +181,37 @@ static void fft3d_full(scadyn_operator *op, cd *A, cd *tmp) {
     const int Lx = op->L[0], Ly = op->L[1], Lz = op->L[2];
     const size_t budget = 64 * 1024;
     LinePass p;
-    make_line_pass(p, Lx, Lx, Lx, 1, (long long)Ly * Lz, budget);
-    launch_line_fft(p, A, tmp, op->d_tw, -1, op->s);
-    make_line_pass(p, Ly, Ly, Ly, Lx, Lz, budget);
-    launch_line_fft(p, tmp, A, op->d_tw + Lx, -1, op->s);
-    make_line_pass(p, Lz, Lz, Lz, (long long)Lx * Ly, 1, budget);
-    launch_line_fft(p, A, tmp, op->d_tw + Lx + Ly, -1, op->s);
+    make_line_pass(p, Lx, Lx, Lx, 1, (long long)Ly * Lz, budget, sizeof(cd));
+    launch_line_fft<cd>(p, A, tmp, op->d_tw, -1, op->s);
+    make_line_pass(p, Ly, Ly, Ly, Lx, Lz, budget, sizeof(cd));
+    launch_line_fft<cd>(p, tmp, A, op->d_tw + Lx, -1, op->s);
+    make_line_pass(p, Lz, Lz, Lz, (long long)Lx * Ly, 1, budget, sizeof(cd));
+    launch_line_fft<cd>(p, A, tmp, op->d_tw + Lx + Ly, -1, op->s);
     HIP_CHECK(hipMemcpyAsync(A, tmp, sizeof(cd) * (size_t)Lx * Ly * Lz, hipMemcpyDeviceToDevice, op->s));
+}
+
+__global__ void k_to_f32(const cd *__restrict__ a, cf *__restrict__ b, size_t n) {
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        b[e] = make_float2((float)a[e].x, (float)a[e].y);
+}
+
+// complex64 copies of S*, FFT(G) and the twiddles for the mixed-precision matvec
+static void make_fp32_copies(scadyn_operator *op) {
+    if (op->d_S4f) return;
+    DevMem &M = op->mem;
+    const size_t nS = (size_t)op->g.N * 108, nF = (size_t)op->Hh[0] * op->Hh[1] * op->Hh[2];
+    op->d_S4f = M.alloc<cf>(nS);
+    op->d_fgf = M.alloc<cf>(nF);
+    hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(256), 0, op->s, op->d_S4, op->d_S4f, nS);
+    hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(256), 0, op->s, op->d_fg, op->d_fgf, nF);
+    HIP_CHECK(hipGetLastError());
+    std::vector<cf> tw;
+    for (int c = 0; c < 3; ++c) {
+        auto t = twiddle_table(op->L[c]);
+        for (auto &v : t) tw.push_back(make_float2((float)v.x, (float)v.y));
+    }
+    op->d_twf = M.upload(tw, op->s);
+    HIP_CHECK(hipStreamSynchronize(op->s));
 }
 
 static void create_operator(scadyn_operator *op, const scadyn_mesh &mesh, const scadyn_opts &opts, double k) {
@@ -314,6 +340,7 @@ static void create_operator(scadyn_operator *op, const scadyn_mesh &mesh, const 
         M.release(A);
         M.release(T);
     }
+    if (opts.precision == SCADYN_PREC_MIXED) make_fp32_copies(op);
 }
 
 // ---------------------------------------------------------------- workspace / matvec
@@ -358,11 +385,14 @@ static Workspace &workspace(scadyn_operator *op, int B) {
     w.col = w.mem.alloc<int>(B);
     w.active = w.mem.alloc<int>(B);
     const size_t budget = 64 * 1024;
-    make_line_pass(w.px, Lx, Nx, Lx, nb, (long long)Ny * Nz, budget);
-    make_line_pass(w.py, Ly, Ny, Ly, (long long)Lx * nb, Nz, budget);
-    make_line_pass(w.pz, Lz, Nz, Nz, (long long)Ly * Lx * nb, 1, budget);
-    make_line_pass(w.qy, Ly, Ly, Ny, (long long)Lx * nb, Nz, budget);
-    make_line_pass(w.qx, Lx, Lx, Nx, nb, (long long)Ny * Nz, budget);
+    for (int pr = 0; pr < 2; ++pr) {
+        const size_t el = pr == 0 ? sizeof(cd) : sizeof(cf);
+        make_line_pass(w.pass[pr][0], Lx, Nx, Lx, nb, (long long)Ny * Nz, budget, el);
+        make_line_pass(w.pass[pr][1], Ly, Ny, Ly, (long long)Lx * nb, Nz, budget, el);
+        make_line_pass(w.pass[pr][2], Lz, Nz, Nz, (long long)Ly * Lx * nb, 1, budget, el);
+        make_line_pass(w.pass[pr][3], Ly, Ly, Ny, (long long)Lx * nb, Nz, budget, el);
+        make_line_pass(w.pass[pr][4], Lx, Lx, Nx, nb, (long long)Ny * Nz, budget, el);
+    }
     return w;
 }
 
@@ -378,10 +408,32 @@ struct ProfScope {
     }
 };
 
-// y = A x for the batch: x element (row, b) at xin[in_off[b] + row*B + b]
+template <typename C>
+struct OpData;
+template <>
+struct OpData<cd> {
+    static const cd *S4(scadyn_operator *op) { return op->d_S4; }
+    static const cd *fg(scadyn_operator *op) { return op->d_fg; }
+    static const cd *tw(scadyn_operator *op) { return op->d_tw; }
+    static const int pr = 0;
+};
+template <>
+struct OpData<cf> {
+    static const cf *S4(scadyn_operator *op) { return op->d_S4f; }
+    static const cf *fg(scadyn_operator *op) { return op->d_fgf; }
+    static const cf *tw(scadyn_operator *op) { return op->d_twf; }
+    static const int pr = 1;
+};
+
+// y = A x for the batch: x element (row, b) at xin[in_off[b] + row*B + b];
+// grids, S* and FFT(G) in precision C, x and y complex128
+template <typename C>
 static void run_matvec(scadyn_operator *op, Workspace &w, const cd *xin, const int *active, cd *y) {
     const HostGeometry &g = op->g;
-    MatvecArgs a;
+    const int pr = OpData<C>::pr;
+    C *g0 = (C *)w.grid0, *g1 = (C *)w.grid1;
+    const C *tw = OpData<C>::tw(op);
+    MatvecArgs<C> a;
     a.xin = xin;
     a.in_off = w.in_off;
     a.B = w.B;
@@ -394,34 +446,35 @@ static void run_matvec(scadyn_operator *op, Workspace &w, const cd *xin, const i
     a.ncz = g.ncz;
     a.cube_start = op->d_cube_start;
     a.tet_cube = op->d_tet_cube;
-    a.S4 = op->d_S4;
+    a.S4 = OpData<C>::S4(op);
     a.row_start = op->d_row_start;
     a.ent_col = op->d_ent_col;
     a.ent_blk = op->d_ent_blk;
     a.blocks = op->d_blocks;
     a.k2 = op->k * op->k;
     a.slot_active = active;
-    a.grid_in = w.grid0;
-    a.grid_out = w.grid0;
+    a.grid_in = g0;
+    a.grid_out = g0;
     a.y = y;
-    const double c = sizeof(cd);
+    const double c = sizeof(C);
     const double nB = 4.0 * w.B;
     const double Lx = op->L[0], Ly = op->L[1], Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+    const int ko = pr ? SCADYN_KCLASS_COUNT : 0;  // fp32 launches are accounted separately
+    (void)ko;
     {
-        ProfScope ps(op, SCADYN_KCLASS_SCATTER, c * (Nx * Ny * Nz * nB + 3.0 * g.N * w.B) + 1728.0 * g.N);
-        launch_scatter(a, op->s);
+        ProfScope ps(op, SCADYN_KCLASS_SCATTER, c * (Nx * Ny * Nz * nB + 27.0 * 4 * g.N) + 16.0 * 3 * g.N * w.B);
+        launch_scatter<C>(a, op->s);
     }
     {
         ProfScope ps(op, SCADYN_KCLASS_FFTX, c * nB * (Nx * Ny * Nz + Lx * Ny * Nz));
-        launch_line_fft(w.px, w.grid0, w.grid1, op->d_tw, -1, op->s);
+        launch_line_fft<C>(w.pass[pr][0], g0, g1, tw, -1, op->s);
     }
     {
         ProfScope ps(op, SCADYN_KCLASS_FFTY, c * nB * (Lx * Ny * Nz + Lx * Ly * Nz));
-        launch_line_fft(w.py, w.grid1, w.grid0, op->d_tw + op->L[0], -1, op->s);
+        launch_line_fft<C>(w.pass[pr][1], g1, g0, tw + op->L[0], -1, op->s);
     }
     {
         ConvInfo ci;
-        ci.fg = op->d_fg;
         ci.Lx = op->L[0];
         ci.Ly = op->L[1];
         ci.Lz = op->L[2];
@@ -430,20 +483,20 @@ static void run_matvec(scadyn_operator *op, Workspace &w, const cd *xin, const i
         ci.nb = 4LL * w.B;
         ProfScope ps(op, SCADYN_KCLASS_FFTZCONV,
                      c * (nB * 2.0 * Lx * Ly * Nz + (double)op->Hh[0] * op->Hh[1] * op->Hh[2]));
-        launch_line_conv(w.pz, w.grid0, w.grid0, op->d_tw + op->L[0] + op->L[1], ci, op->s);
+        launch_line_conv<C>(w.pass[pr][2], g0, g0, tw + op->L[0] + op->L[1], OpData<C>::fg(op), ci, op->s);
     }
     {
         ProfScope ps(op, SCADYN_KCLASS_IFFTY, c * nB * (Lx * Ly * Nz + Lx * Ny * Nz));
-        launch_line_fft(w.qy, w.grid0, w.grid1, op->d_tw + op->L[0], +1, op->s);
+        launch_line_fft<C>(w.pass[pr][3], g0, g1, tw + op->L[0], +1, op->s);
     }
     {
         ProfScope ps(op, SCADYN_KCLASS_IFFTX, c * nB * (Lx * Ny * Nz + Nx * Ny * Nz));
-        launch_line_fft(w.qx, w.grid1, w.grid0, op->d_tw, +1, op->s);
+        launch_line_fft<C>(w.pass[pr][4], g1, g0, tw, +1, op->s);
     }
     {
         ProfScope ps(op, SCADYN_KCLASS_GATHER,
-                     c * (Nx * Ny * Nz * nB + 6.0 * g.N * w.B) + 1728.0 * g.N + 80.0 * op->nblocks);
-        launch_gather_near(a, op->s);
+                     c * (Nx * Ny * Nz * nB + 27.0 * 4 * g.N) + 16.0 * 6 * g.N * w.B + 80.0 * op->nblocks);
+        launch_gather_near<C>(a, op->s);
     }
 }
 
@@ -481,6 +534,8 @@ static void h2d_sync(void *dst, const void *src, size_t bytes, hipStream_t s) {
 
 static int auto_batch(scadyn_operator *op, int ncols) {
     if (op->opts.batch > 0) return std::min(op->opts.batch, std::max(ncols, 1));
+    // an existing workspace that holds enough slots is reused (no reallocation)
+    if (op->ws && op->ws->m == op->opts.maxit && op->ws->B >= std::min(ncols, 64)) return op->ws->B;
     size_t free_b = 0, total_b = 0;
     HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
     size_t per = slot_bytes(op, op->opts.maxit);
@@ -491,9 +546,20 @@ static int auto_batch(scadyn_operator *op, int ncols) {
     return B;
 }
 
-// Solve for the columns of src (n x ld, internal rows); solutions into dst.
-static void solve_columns(scadyn_operator *op, const cd *src, long long ld, int ncols, cd *dst, int32_t *iters,
-                          int *nconv_fail) {
+struct SolveSpec {
+    int prec = SCADYN_PREC_FP64;  // matvec precision
+    bool x0zero = false;          // x0 = 0 (correction equations) instead of x0 = rhs
+    double tol = 1e-5;
+    int restart = 4;
+};
+
+// Solve A x = src(:, c) for every column c of `cols` (src, dst: n x ld, internal
+// rows); continuous batching: a slot is refilled as soon as its system finishes.
+static void solve_columns(scadyn_operator *op, const cd *src, long long ld, const std::vector<int> &cols, cd *dst,
+                          int32_t *iters, int *nconv_fail, const SolveSpec &spec) {
+    const int ncols = (int)cols.size();
+    if (ncols == 0) return;
+    const int prec = spec.prec;
     const int B = auto_batch(op, ncols);
     Workspace &w = workspace(op, B);
     hipStream_t s = op->s;
@@ -510,8 +576,7 @@ static void solve_columns(scadyn_operator *op, const cd *src, long long ld, int 
     HIP_CHECK(hipMemsetAsync(w.st, 0, sizeof(SlotState) * B, s));
     HIP_CHECK(hipMemsetAsync(w.V, 0, sizeof(cd) * (size_t)(w.m + 2) * nB, s));
     // every slot finishes within restart*(m+1) matvecs; bound the loop
-    const long long max_steps =
-        ((long long)op->opts.restart * (w.m + 1) + 4) * ((ncols + B - 1) / B + 2) + 16;
+    const long long max_steps = ((long long)spec.restart * (w.m + 1) + 4) * ((ncols + B - 1) / B + 2) + 16;
     long long steps = 0;
     for (;;) {
         if (++steps > max_steps) fail(SCADYN_ERR_DEVICE, "GMRES driver made no progress");
@@ -520,19 +585,18 @@ static void solve_columns(scadyn_operator *op, const cd *src, long long ld, int 
         for (int b = 0; b < B; ++b) {
             loadc[b] = -1;
             if (slot_col[b] < 0 && next < ncols) {
-                loadc[b] = next++;
+                loadc[b] = cols[next++];
                 slot_col[b] = loadc[b];
                 any_load = true;
             }
         }
         if (any_load) {
             h2d_sync(w.col, loadc.data(), sizeof(int) * B, s);
-            launch_load_rhs(d, w.col, src, ld, w.Rhs, w.V, w.pnrm, s);
+            launch_load_rhs(d, w.col, src, ld, w.Rhs, w.V, w.pnrm, spec.x0zero ? 1 : 0, s);
             launch_load_finish(d, w.col, w.st, w.pnrm, s);
         }
         HIP_CHECK(hipMemcpyAsync(hs.data(), w.st, sizeof(SlotState) * B, hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
-        // slots finished at load (zero rhs)
         bool busy = false;
         int K = -1;
         for (int b = 0; b < B; ++b) {
@@ -554,15 +618,17 @@ static void solve_columns(scadyn_operator *op, const cd *src, long long ld, int 
         if (busy) {
             h2d_sync(w.in_off, off.data(), sizeof(long long) * B, s);
             h2d_sync(w.active, act.data(), sizeof(int) * B, s);
-            run_matvec(op, w, w.V, w.active, w.W);
+            if (prec == SCADYN_PREC_MIXED)
+                run_matvec<cf>(op, w, w.V, w.active, w.W);
+            else
+                run_matvec<cd>(op, w, w.V, w.active, w.W);
             {
                 ProfScope ps(op, SCADYN_KCLASS_KRYLOV, 0.0);
                 for (int kk = 0; kk <= K; ++kk) {
                     launch_mgs_pass(d, kk, w.W, w.V, w.Rhs, w.st, w.H, w.pdot, w.pnrm, w.pnav, s);
                     launch_mgs_reduce(d, kk, w.st, w.H, w.pdot, w.pnrm, w.pnav, w.nrm2, w.nav2, s);
                 }
-                launch_slot_update(d, w.st, w.H, w.cs, w.sn, w.g, w.y, w.nrm2, w.nav2, op->opts.tol,
-                                   op->opts.restart, s);
+                launch_slot_update(d, w.st, w.H, w.cs, w.sn, w.g, w.y, w.nrm2, w.nav2, spec.tol, spec.restart, s);
                 launch_vec_update(d, w.st, w.W, w.V, w.y, s);
             }
             HIP_CHECK(hipMemcpyAsync(hs.data(), w.st, sizeof(SlotState) * B, hipMemcpyDeviceToHost, s));
@@ -575,7 +641,7 @@ static void solve_columns(scadyn_operator *op, const cd *src, long long ld, int 
             loadc[b] = -1;
             if (slot_col[b] >= 0 && hs[b].phase == PH_DONE) {
                 loadc[b] = slot_col[b];
-                if (iters) iters[slot_col[b]] = hs[b].ite;
+                if (iters) iters[slot_col[b]] += hs[b].ite;
                 if (hs[b].conv == 0) nfail++;
                 if (hs[b].conv < 0) singular++;
                 slot_col[b] = -1;
@@ -597,7 +663,127 @@ static void solve_columns(scadyn_operator *op, const cd *src, long long ld, int 
     }
     HIP_CHECK(hipStreamSynchronize(s));
     if (singular) fail(SCADYN_ERR_SINGULAR, "Matrix is numerically singular! (GMRES least squares)");
-    if (nconv_fail) *nconv_fail = nfail;
+    if (nconv_fail) *nconv_fail += nfail;
+}
+
+// ---- column utilities for the refinement loop (n x ld row-major, internal rows)
+__global__ void k_col_norm2(const cd *__restrict__ A, long long n, int ld, double *__restrict__ part) {
+    // part[blockIdx.x][col]; threads stride over rows for one column each
+    for (int c = threadIdx.x; c < ld; c += blockDim.x) {
+        double acc = 0.0;
+        for (long long r = blockIdx.x; r < n; r += gridDim.x) acc += cabs2(A[(size_t)r * ld + c]);
+        part[(size_t)blockIdx.x * ld + c] = acc;
+    }
+}
+__global__ void k_add_cols(cd *__restrict__ X, const cd *__restrict__ D, long long n, int ld,
+                           const int *__restrict__ mask) {
+    const long long tot = n * ld;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
+         e += (long long)gridDim.x * blockDim.x)
+        if (mask[e % ld]) X[e] = cadd(X[e], D[e]);
+}
+// gather columns cols[b] of X into the batch layout [n][B]
+__global__ void k_gather_cols(const cd *__restrict__ X, long long n, int ld, const int *__restrict__ cols, int B,
+                              cd *__restrict__ out) {
+    const long long tot = n * B;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int b = (int)(e % B);
+        const long long r = e / B;
+        out[e] = cols[b] >= 0 ? X[(size_t)r * ld + cols[b]] : cmk(0, 0);
+    }
+}
+// R(:, cols[b]) = Bm(:, cols[b]) - AX(:, b)
+__global__ void k_resid_cols(const cd *__restrict__ Bm, const cd *__restrict__ AX, long long n, int ld,
+                             const int *__restrict__ cols, int B, cd *__restrict__ R) {
+    const long long tot = n * B;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int b = (int)(e % B);
+        if (cols[b] < 0) continue;
+        const long long r = e / B;
+        const size_t o = (size_t)r * ld + cols[b];
+        R[o] = csub(Bm[o], AX[e]);
+    }
+}
+
+static std::vector<double> col_norms(scadyn_operator *op, const cd *A, long long n, int ld) {
+    const int nb = 256;
+    DevMem tmp;
+    double *part = tmp.alloc<double>((size_t)nb * ld);
+    hipLaunchKernelGGL(k_col_norm2, dim3(nb), dim3(256), 0, op->s, A, n, ld, part);
+    std::vector<double> h((size_t)nb * ld), out(ld, 0.0);
+    HIP_CHECK(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, op->s));
+    HIP_CHECK(hipStreamSynchronize(op->s));
+    for (int b = 0; b < nb; ++b)
+        for (int c = 0; c < ld; ++c) out[c] += h[(size_t)b * ld + c];
+    for (auto &v : out) v = std::sqrt(v);
+    return out;
+}
+
+// R(:, c) = Bm(:, c) - A X(:, c) with the FP64 operator, for c in cols
+static void fp64_residual(scadyn_operator *op, const cd *Bm, const cd *X, long long n, int ld,
+                          const std::vector<int> &cols, cd *R) {
+    const int B = auto_batch(op, (int)cols.size());
+    Workspace &w = workspace(op, B);
+    std::vector<long long> off(B, 0);
+    h2d_sync(w.in_off, off.data(), sizeof(long long) * B, op->s);
+    for (size_t c0 = 0; c0 < cols.size(); c0 += B) {
+        std::vector<int> cb(B, -1);
+        for (int b = 0; b < B && c0 + b < cols.size(); ++b) cb[b] = cols[c0 + b];
+        h2d_sync(w.col, cb.data(), sizeof(int) * B, op->s);
+        hipLaunchKernelGGL(k_gather_cols, dim3(nblk(n * B)), dim3(256), 0, op->s, X, n, ld, w.col, B, w.Rhs);
+        run_matvec<cd>(op, w, w.Rhs, nullptr, w.W);
+        hipLaunchKernelGGL(k_resid_cols, dim3(nblk(n * B)), dim3(256), 0, op->s, Bm, w.W, n, ld, w.col, B, R);
+    }
+    HIP_CHECK(hipStreamSynchronize(op->s));
+}
+
+// Solve A X = Bm for all ld columns.  FP64: the reference's GMRES.  Mixed:
+// iterative refinement, inner GMRES on the FP32 matvec from x0 = 0 to
+// opts.tol, outer FP64 residual until ||b - A x|| <= refine_tol ||b||.
+static void solve_all(scadyn_operator *op, const cd *Bm, long long n, int ld, cd *X, int32_t *iters, int *nfail,
+                      int32_t *outer_rounds) {
+    std::vector<int> all(ld);
+    for (int c = 0; c < ld; ++c) all[c] = c;
+    if (iters)
+        for (int c = 0; c < ld; ++c) iters[c] = 0;
+    SolveSpec spec;
+    spec.tol = op->opts.tol;
+    spec.restart = op->opts.restart;
+    if (op->opts.precision != SCADYN_PREC_MIXED) {
+        solve_columns(op, Bm, ld, all, X, iters, nfail, spec);
+        if (outer_rounds) *outer_rounds = 1;
+        return;
+    }
+    spec.prec = SCADYN_PREC_MIXED;
+    spec.x0zero = true;
+    DevMem tmp;
+    cd *R = tmp.alloc<cd>((size_t)n * ld);
+    cd *D = tmp.alloc<cd>((size_t)n * ld);
+    int *mask = tmp.alloc<int>(ld);
+    HIP_CHECK(hipMemsetAsync(X, 0, sizeof(cd) * (size_t)n * ld, op->s));
+    HIP_CHECK(hipMemcpyAsync(R, Bm, sizeof(cd) * (size_t)n * ld, hipMemcpyDeviceToDevice, op->s));
+    std::vector<double> bn = col_norms(op, Bm, n, ld);
+    std::vector<int> active = all;
+    int rounds = 0;
+    for (int outer = 0; outer < 30 && !active.empty(); ++outer) {
+        ++rounds;
+        int nf = 0;
+        solve_columns(op, R, ld, active, D, iters, &nf, spec);
+        std::vector<int> m(ld, 0);
+        for (int c : active) m[c] = 1;
+        h2d_sync(mask, m.data(), sizeof(int) * ld, op->s);
+        hipLaunchKernelGGL(k_add_cols, dim3(nblk(n * ld)), dim3(256), 0, op->s, X, D, n, ld, mask);
+        fp64_residual(op, Bm, X, n, ld, active, R);
+        std::vector<double> rn = col_norms(op, R, n, ld);
+        std::vector<int> still;
+        for (int c : active)
+            if (!(rn[c] <= op->opts.refine_tol * bn[c])) still.push_back(c);
+        active.swap(still);
+    }
+    if (nfail) *nfail += (int)active.size();
+    if (outer_rounds) *outer_rounds = rounds;
 }
 
 static void tmatrix(scadyn_operator *op, int nmax, double *Taa, double *Tab, double *Tba, double *Tbb,
@@ -611,7 +797,7 @@ static void tmatrix(scadyn_operator *op, int nmax, double *Taa, double *Tab, dou
     launch_vswf(op->d_tcoord, op->d_vol, op->g.N, nmax, op->k, mat, ncol, s);
     cd *X = tmp.alloc<cd>((size_t)n * ncol);
     int nfail = 0;
-    solve_columns(op, mat, ncol, ncol, X, iters, &nfail);
+    solve_all(op, mat, n, ncol, X, iters, &nfail, &op->last_outer_rounds);
     const size_t work = std::max<size_t>((size_t)ncol * ncol * 64, 1 << 20);
     cd *wk = tmp.alloc<cd>(work);
     cd *T = tmp.alloc<cd>((size_t)ncol * ncol);
@@ -682,7 +868,8 @@ int scadyn_op_create(const scadyn_mesh *mesh, const scadyn_opts *opts, double k,
         o = *opts;
     else
         scadyn_opts_default(&o);
-    if (o.precision != SCADYN_PREC_FP64) fail(SCADYN_ERR_UNSUPPORTED, "only the FP64 path is built");
+    if (o.precision != SCADYN_PREC_FP64 && o.precision != SCADYN_PREC_MIXED)
+        fail(SCADYN_ERR_ARG, "unknown precision mode");
     if (o.maxit < 1 || o.restart < 1) fail(SCADYN_ERR_ARG, "maxit and restart must be >= 1");
     *out = nullptr;
     std::unique_ptr<scadyn_operator> op(new scadyn_operator());
@@ -759,10 +946,12 @@ int scadyn_op_export(const scadyn_operator *opc, int32_t what, void *host, int64
     ABI_CATCH
 }
 
-int scadyn_op_matvec(scadyn_operator *op, int32_t nrhs, const double *x, double *y) {
+int scadyn_op_matvec_prec(scadyn_operator *op, int32_t precision, int32_t nrhs, const double *x, double *y) {
     ABI_TRY
     if (!op || !x || !y || nrhs < 1) fail(SCADYN_ERR_ARG, "bad argument");
+    if (precision != SCADYN_PREC_FP64 && precision != SCADYN_PREC_MIXED) fail(SCADYN_ERR_ARG, "bad precision");
     HIP_CHECK(hipSetDevice(op->device));
+    if (precision == SCADYN_PREC_MIXED) make_fp32_copies(op);
     const int N = op->g.N;
     Workspace &w = workspace(op, nrhs);
     hipStream_t s = op->s;
@@ -774,12 +963,19 @@ int scadyn_op_matvec(scadyn_operator *op, int32_t nrhs, const double *x, double 
     hipLaunchKernelGGL(k_pack, dim3(nblk(3LL * N * nrhs)), dim3(256), 0, s, hx, nrhs, N, op->d_perm, nrhs, xin);
     std::vector<long long> off(nrhs, 0);
     h2d_sync(w.in_off, off.data(), sizeof(long long) * nrhs, s);
-    run_matvec(op, w, xin, nullptr, w.W);
+    if (precision == SCADYN_PREC_MIXED)
+        run_matvec<cf>(op, w, xin, nullptr, w.W);
+    else
+        run_matvec<cd>(op, w, xin, nullptr, w.W);
     hipLaunchKernelGGL(k_unpack, dim3(nblk(3LL * N * nrhs)), dim3(256), 0, s, w.W, nrhs, N, op->d_perm, nrhs, hx);
     HIP_CHECK(hipMemcpyAsync(y, hx, sizeof(cd) * nv, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     if (op->prof.on) op->prof.collect();
     ABI_CATCH
+}
+
+int scadyn_op_matvec(scadyn_operator *op, int32_t nrhs, const double *x, double *y) {
+    return scadyn_op_matvec_prec(op, SCADYN_PREC_FP64, nrhs, x, y);
 }
 
 int scadyn_op_solve(scadyn_operator *op, int32_t nrhs, const double *rhs, double *x, int32_t *iters) {
@@ -796,7 +992,8 @@ int scadyn_op_solve(scadyn_operator *op, int32_t nrhs, const double *rhs, double
     HIP_CHECK(hipMemcpyAsync(hx, rhs, sizeof(cd) * nv, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_pack, dim3(nblk(3LL * N * nrhs)), dim3(256), 0, s, hx, nrhs, N, op->d_perm, nrhs, src);
     int nfail = 0;
-    solve_columns(op, src, nrhs, nrhs, dst, iters, &nfail);
+    int32_t rounds = 0;
+    solve_all(op, src, 3LL * N, nrhs, dst, iters, &nfail, &rounds);
     hipLaunchKernelGGL(k_unpack, dim3(nblk(3LL * N * nrhs)), dim3(256), 0, s, dst, nrhs, N, op->d_perm, nrhs, hx);
     HIP_CHECK(hipMemcpyAsync(x, hx, sizeof(cd) * nv, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));

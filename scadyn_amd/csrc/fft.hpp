@@ -25,26 +25,20 @@ struct LinePass {
 // Fg lookup for the fused z-pass: inner index i of the [Ly][Lx][nb] layout
 // -> frequency (kx, ky); kz is the line index.  Fg holds the even octant.
 struct ConvInfo {
-    const cd *fg = nullptr;
     int Lx = 0, Ly = 0, Lz = 0, Hx = 0, Hy = 0;
     long long nb = 0;  // batch columns per grid point
 };
 
-// twiddle tables for lengths used by one operator (device memory)
-struct Twiddles {
-    cd *dev = nullptr;   // concatenated tables exp(-2 pi i m / L), m < L
-    int offset[3] = {0, 0, 0};
-};
+void make_line_pass(LinePass &p, int L, int n_in, int n_out, long long inner, long long outer, size_t lds_budget,
+                    size_t elem);
+size_t line_pass_lds(const LinePass &p, size_t elem);
 
-void make_line_pass(LinePass &p, int L, int n_in, int n_out, long long inner, long long outer,
-                    size_t lds_budget);
-size_t line_pass_lds(const LinePass &p);
-
-// mode 0: forward (sign -1) or inverse (sign +1) transform
-void launch_line_fft(const LinePass &p, const cd *in, cd *out, const cd *tw, int sign,
-                     hipStream_t s);
-// mode 1: forward, multiply by Fg, inverse (the z-pass of the convolution)
-void launch_line_conv(const LinePass &p, const cd *in, cd *out, const cd *tw,
-                      const ConvInfo &ci, hipStream_t s);
+// forward (sign -1) or inverse (sign +1) transform; tw = exp(-2 pi i m / L)
+template <typename C>
+void launch_line_fft(const LinePass &p, const C *in, C *out, const C *tw, int sign, hipStream_t s);
+// forward, multiply by the Fg octant, inverse (the z-pass of the convolution)
+template <typename C>
+void launch_line_conv(const LinePass &p, const C *in, C *out, const C *tw, const C *fg, const ConvInfo &ci,
+                      hipStream_t s);
 
 }  // namespace scadyn

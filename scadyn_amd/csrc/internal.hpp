@@ -56,6 +56,37 @@ __host__ __device__ inline void cfma(cd &acc, cd a, cd b) {
     acc.y = fma(a.x, b.y, fma(a.y, b.x, acc.y));
 }
 
+// single-precision complex (the mixed-precision matvec)
+__host__ __device__ inline cf cmkf(float r, float i) { return make_float2(r, i); }
+__host__ __device__ inline cf cadd(cf a, cf b) { return make_float2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline cf csub(cf a, cf b) { return make_float2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline cf cmul(cf a, cf b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ inline cf cscale(cf a, float s) { return make_float2(a.x * s, a.y * s); }
+__host__ __device__ inline void cfma(cf &acc, cf a, cf b) {
+    acc.x = fmaf(a.x, b.x, fmaf(-a.y, b.y, acc.x));
+    acc.y = fmaf(a.x, b.y, fmaf(a.y, b.x, acc.y));
+}
+
+// precision traits: real type and conversions
+template <typename C>
+struct Prec;
+template <>
+struct Prec<cd> {
+    typedef double real;
+    __host__ __device__ static cd from(cd a) { return a; }
+    __host__ __device__ static cd from(cf a) { return make_double2(a.x, a.y); }
+    __host__ __device__ static cd zero() { return make_double2(0.0, 0.0); }
+};
+template <>
+struct Prec<cf> {
+    typedef float real;
+    __host__ __device__ static cf from(cd a) { return make_float2((float)a.x, (float)a.y); }
+    __host__ __device__ static cf from(cf a) { return a; }
+    __host__ __device__ static cf zero() { return make_float2(0.0f, 0.0f); }
+};
+
 // ---------------------------------------------------------------- geometry (host)
 struct HostGeometry {
     int N = 0, Nnode = 0;

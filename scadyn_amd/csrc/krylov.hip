@@ -236,7 +236,7 @@ __global__ __launch_bounds__(256) void k_vec_update(KrylovDims d, const SlotStat
 __global__ __launch_bounds__(256) void k_load_rhs(KrylovDims d, const int *__restrict__ col,
                                                   const cd *__restrict__ src, long long ld,
                                                   cd *__restrict__ Rhs, cd *__restrict__ V,
-                                                  double *__restrict__ pnrm) {
+                                                  double *__restrict__ pnrm, int x0zero) {
     __shared__ double snrm[256];
     RowSlotMap m = row_slot_map(d.B);
     double nrm = 0.0;
@@ -247,7 +247,8 @@ __global__ __launch_bounds__(256) void k_load_rhs(KrylovDims d, const int *__res
             const size_t e = (size_t)row * d.B + m.b;
             cd v = src[(size_t)row * ld + c];
             Rhs[e] = v;
-            V[(size_t)(d.m + 1) * nB + e] = v;  // x0 = rhs (src/gmres_module.f90:198)
+            // x0 = rhs as the reference (src/gmres_module.f90:198), or 0 for correction equations
+            V[(size_t)(d.m + 1) * nB + e] = x0zero ? cmk(0, 0) : v;
             nrm += cabs2(v);
         }
     }
@@ -367,8 +368,8 @@ void launch_vec_update(const KrylovDims &d, const SlotState *st, const cd *W, cd
     HIP_CHECK(hipGetLastError());
 }
 void launch_load_rhs(const KrylovDims &d, const int *col, const cd *src, long long ld, cd *Rhs, cd *V,
-                     double *pnrm, hipStream_t s) {
-    hipLaunchKernelGGL(k_load_rhs, rs_grid(d), dim3(256), 0, s, d, col, src, ld, Rhs, V, pnrm);
+                     double *pnrm, int x0zero, hipStream_t s) {
+    hipLaunchKernelGGL(k_load_rhs, rs_grid(d), dim3(256), 0, s, d, col, src, ld, Rhs, V, pnrm, x0zero);
     HIP_CHECK(hipGetLastError());
 }
 void launch_load_finish(const KrylovDims &d, const int *col, SlotState *st, const double *pnrm,

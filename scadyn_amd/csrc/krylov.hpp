@@ -45,7 +45,7 @@ void launch_vec_update(const KrylovDims &d, const SlotState *st, const cd *W, cd
                        hipStream_t s);
 // load column col[b] (>= 0) of src (n x ld, row-major) into Rhs and X of slot b
 void launch_load_rhs(const KrylovDims &d, const int *col, const cd *src, long long ld, cd *Rhs, cd *V,
-                     double *pnrm, hipStream_t s);
+                     double *pnrm, int x0zero, hipStream_t s);
 void launch_load_finish(const KrylovDims &d, const int *col, SlotState *st, const double *pnrm,
                         hipStream_t s);
 // store X of slots with col[b] >= 0 into column col[b] of dst (n x ld)

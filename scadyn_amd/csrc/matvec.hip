@@ -12,11 +12,12 @@ namespace scadyn {
 
 // scatter (sparse_T_matmul, src/sparse.f90:13-43, + vec2arr2), node-centric:
 // node p sums the stencil weights of the tets of the <= 8 cubes whose box holds p.
+template <typename C>
 __global__ __launch_bounds__(256) void k_scatter(const cd *__restrict__ xin,
                                                  const long long *__restrict__ in_off, int B,
                                                  const int *__restrict__ cube_start,
-                                                 const cd *__restrict__ S4, int Nx, int Ny, int Nz,
-                                                 int ncx, int ncy, int ncz, cd *__restrict__ grid) {
+                                                 const C *__restrict__ S4, int Nx, int Ny, int Nz,
+                                                 int ncx, int ncy, int ncz, C *__restrict__ grid) {
     const long long tot = (long long)Nx * Ny * Nz * B;
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
          e += (long long)gridDim.x * blockDim.x) {
@@ -24,7 +25,7 @@ __global__ __launch_bounds__(256) void k_scatter(const cd *__restrict__ xin,
         const long long p = e / B;
         const int px = (int)(p % Nx), py = (int)((p / Nx) % Ny), pz = (int)(p / ((long long)Nx * Ny));
         const cd *xb = xin + in_off[b] + b;
-        cd a0 = cmk(0, 0), a1 = cmk(0, 0), a2 = cmk(0, 0), a3 = cmk(0, 0);
+        C a0 = Prec<C>::zero(), a1 = a0, a2 = a0, a3 = a0;
         // cubes c with 2c <= p <= 2c+2 per axis
         const int cx0 = max(0, (px - 1) / 2), cx1 = min(ncx - 1, px / 2);
         const int cy0 = max(0, (py - 1) / 2), cy1 = min(ncy - 1, py / 2);
@@ -35,10 +36,11 @@ __global__ __launch_bounds__(256) void k_scatter(const cd *__restrict__ xin,
                     const int c = cx + ncx * (cy + ncy * cz);
                     const int j = (px - 2 * cx) + 3 * (py - 2 * cy) + 9 * (pz - 2 * cz);
                     for (int t = cube_start[c]; t < cube_start[c + 1]; ++t) {
-                        const cd *s = S4 + ((size_t)t * 27 + j) * 4;
-                        const cd x0 = xb[(size_t)(3 * t) * B], x1 = xb[(size_t)(3 * t + 1) * B],
-                                 x2 = xb[(size_t)(3 * t + 2) * B];
-                        const cd s0 = s[0];
+                        const C *s = S4 + ((size_t)t * 27 + j) * 4;
+                        const C x0 = Prec<C>::from(xb[(size_t)(3 * t) * B]),
+                                x1 = Prec<C>::from(xb[(size_t)(3 * t + 1) * B]),
+                                x2 = Prec<C>::from(xb[(size_t)(3 * t + 2) * B]);
+                        const C s0 = s[0];
                         cfma(a0, s0, x0);
                         cfma(a1, s0, x1);
                         cfma(a2, s0, x2);
@@ -47,7 +49,7 @@ __global__ __launch_bounds__(256) void k_scatter(const cd *__restrict__ xin,
                         cfma(a3, s[3], x2);
                     }
                 }
-        cd *gp = grid + (size_t)p * 4 * B + b;
+        C *gp = grid + (size_t)p * 4 * B + b;
         gp[0] = a0;
         gp[(size_t)B] = a1;
         gp[(size_t)2 * B] = a2;
@@ -57,9 +59,10 @@ __global__ __launch_bounds__(256) void k_scatter(const cd *__restrict__ xin,
 
 // gather (sparse_eps_matmul, src/sparse.f90:47-77) + Y_aim = Y2 - k^2 Y (:148-153)
 // + symmetric near-zone blocks (matmul_Acorr_const, src/sparse_mat.f90:96-123).
+template <typename C>
 __global__ __launch_bounds__(256) void k_gather_near(
-    const cd *__restrict__ grid, const cd *__restrict__ xin, const long long *__restrict__ in_off, int B,
-    int N, const int *__restrict__ tet_cube, const cd *__restrict__ S4, int Nx, int Ny, int ncx, int ncy,
+    const C *__restrict__ grid, const cd *__restrict__ xin, const long long *__restrict__ in_off, int B,
+    int N, const int *__restrict__ tet_cube, const C *__restrict__ S4, int Nx, int Ny, int ncx, int ncy,
     const int64_t *__restrict__ row_start, const int *__restrict__ ent_col,
     const uint32_t *__restrict__ ent_blk, const cf *__restrict__ blocks, double k2,
     const int *__restrict__ slot_active, cd *__restrict__ y) {
@@ -76,17 +79,16 @@ __global__ __launch_bounds__(256) void k_gather_near(
         const int c = tet_cube[t];
         const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
         const size_t base = (size_t)(2 * cx) + (size_t)Nx * ((size_t)(2 * cy) + (size_t)Ny * (2 * cz));
-        const cd *s = S4 + (size_t)t * 108;
-        cd Y0 = cmk(0, 0), Y1 = cmk(0, 0), Y2 = cmk(0, 0);
-        cd Z0 = cmk(0, 0), Z1 = cmk(0, 0), Z2 = cmk(0, 0);
+        const C *s = S4 + (size_t)t * 108;
+        C Y0 = Prec<C>::zero(), Y1 = Y0, Y2 = Y0, Z0 = Y0, Z1 = Y0, Z2 = Y0;
         for (int jz = 0; jz < 3; ++jz)
             for (int jy = 0; jy < 3; ++jy)
                 for (int jx = 0; jx < 3; ++jx) {
                     const int j = jx + 3 * jy + 9 * jz;
                     const size_t node = base + jx + (size_t)Nx * (jy + (size_t)Ny * jz);
-                    const cd *g = grid + node * 4 * B + b;
-                    const cd g0 = g[0], g1 = g[(size_t)B], g2 = g[(size_t)2 * B], g3 = g[(size_t)3 * B];
-                    const cd *sj = s + j * 4;
+                    const C *g = grid + node * 4 * B + b;
+                    const C g0 = g[0], g1 = g[(size_t)B], g2 = g[(size_t)2 * B], g3 = g[(size_t)3 * B];
+                    const C *sj = s + j * 4;
                     cfma(Y0, sj[0], g0);
                     cfma(Y1, sj[0], g1);
                     cfma(Y2, sj[0], g2);
@@ -96,16 +98,17 @@ __global__ __launch_bounds__(256) void k_gather_near(
                 }
         // near zone
         const cd *xb = xin + in_off[b] + b;
-        cd n0 = cmk(0, 0), n1 = cmk(0, 0), n2 = cmk(0, 0);
+        C n0 = Prec<C>::zero(), n1 = n0, n2 = n0;
         for (int64_t q = row_start[t]; q < row_start[t + 1]; ++q) {
             const int col = ent_col[q];
             const uint32_t eb = ent_blk[q];
             const cf *a = blocks + (size_t)(eb & 0x7fffffffu) * 9;
-            const cd x0 = xb[(size_t)(3 * col) * B], x1 = xb[(size_t)(3 * col + 1) * B],
-                     x2 = xb[(size_t)(3 * col + 2) * B];
-            cd m[9];
+            const C x0 = Prec<C>::from(xb[(size_t)(3 * col) * B]),
+                    x1 = Prec<C>::from(xb[(size_t)(3 * col + 1) * B]),
+                    x2 = Prec<C>::from(xb[(size_t)(3 * col + 2) * B]);
+            C m[9];
 #pragma unroll
-            for (int i = 0; i < 9; ++i) m[i] = cmk(a[i].x, a[i].y);
+            for (int i = 0; i < 9; ++i) m[i] = Prec<C>::from(a[i]);
             if (eb & 0x80000000u) {
                 // transpose: y_i += sum_j A(j,i) x_j ; A(i,j) at i + 3j
                 cfma(n0, m[0], x0); cfma(n0, m[1], x1); cfma(n0, m[2], x2);
@@ -117,9 +120,11 @@ __global__ __launch_bounds__(256) void k_gather_near(
                 cfma(n2, m[2], x0); cfma(n2, m[5], x1); cfma(n2, m[8], x2);
             }
         }
-        yo[0] = cadd(n0, csub(Z0, cscale(Y0, k2)));
-        yo[(size_t)B] = cadd(n1, csub(Z1, cscale(Y1, k2)));
-        yo[(size_t)2 * B] = cadd(n2, csub(Z2, cscale(Y2, k2)));
+        typedef typename Prec<C>::real R;
+        const R kk = (R)k2;
+        yo[0] = Prec<cd>::from(cadd(n0, csub(Z0, cscale(Y0, kk))));
+        yo[(size_t)B] = Prec<cd>::from(cadd(n1, csub(Z1, cscale(Y1, kk))));
+        yo[(size_t)2 * B] = Prec<cd>::from(cadd(n2, csub(Z2, cscale(Y2, kk))));
     }
 }
 
@@ -128,19 +133,26 @@ static unsigned grid_for(long long tot, int bs) {
     return (unsigned)std::min<long long>(std::max<long long>(g, 1), 1LL << 20);
 }
 
-void launch_scatter(const MatvecArgs &a, hipStream_t s) {
+template <typename C>
+void launch_scatter(const MatvecArgs<C> &a, hipStream_t s) {
     long long tot = (long long)a.Nx * a.Ny * a.Nz * a.B;
-    hipLaunchKernelGGL(k_scatter, dim3(grid_for(tot, 256)), dim3(256), 0, s, a.xin, a.in_off, a.B,
+    hipLaunchKernelGGL((k_scatter<C>), dim3(grid_for(tot, 256)), dim3(256), 0, s, a.xin, a.in_off, a.B,
                        a.cube_start, a.S4, a.Nx, a.Ny, a.Nz, a.ncx, a.ncy, a.ncz, a.grid_in);
     HIP_CHECK(hipGetLastError());
 }
 
-void launch_gather_near(const MatvecArgs &a, hipStream_t s) {
+template <typename C>
+void launch_gather_near(const MatvecArgs<C> &a, hipStream_t s) {
     long long tot = (long long)a.N * a.B;
-    hipLaunchKernelGGL(k_gather_near, dim3(grid_for(tot, 256)), dim3(256), 0, s, a.grid_out, a.xin, a.in_off,
-                       a.B, a.N, a.tet_cube, a.S4, a.Nx, a.Ny, a.ncx, a.ncy, a.row_start, a.ent_col, a.ent_blk,
-                       a.blocks, a.k2, a.slot_active, a.y);
+    hipLaunchKernelGGL((k_gather_near<C>), dim3(grid_for(tot, 256)), dim3(256), 0, s, a.grid_out, a.xin,
+                       a.in_off, a.B, a.N, a.tet_cube, a.S4, a.Nx, a.Ny, a.ncx, a.ncy, a.row_start, a.ent_col,
+                       a.ent_blk, a.blocks, a.k2, a.slot_active, a.y);
     HIP_CHECK(hipGetLastError());
 }
+
+template void launch_scatter<cd>(const MatvecArgs<cd> &, hipStream_t);
+template void launch_scatter<cf>(const MatvecArgs<cf> &, hipStream_t);
+template void launch_gather_near<cd>(const MatvecArgs<cd> &, hipStream_t);
+template void launch_gather_near<cf>(const MatvecArgs<cf> &, hipStream_t);
 
 }  // namespace scadyn

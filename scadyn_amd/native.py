@@ -19,7 +19,7 @@ KCLASS = ['scatter', 'fftx', 'ffty', 'fftz_conv', 'iffty', 'ifftx', 'gather_near
 
 # every symbol include/scadyn_amd.h declares
 EXPORTED = ['scadyn_opts_default', 'scadyn_last_error', 'scadyn_tmat_solve', 'scadyn_op_create',
-            'scadyn_op_destroy', 'scadyn_op_info', 'scadyn_op_export', 'scadyn_op_matvec',
+            'scadyn_op_destroy', 'scadyn_op_info', 'scadyn_op_export', 'scadyn_op_matvec', 'scadyn_op_matvec_prec',
             'scadyn_op_solve', 'scadyn_op_vswf', 'scadyn_op_tmatrix', 'scadyn_op_profile',
             'scadyn_op_timings']
 
@@ -71,6 +71,7 @@ def lib():
     L.scadyn_op_info.argtypes = [P, ctypes.POINTER(ctypes.c_int64), dp]
     L.scadyn_op_export.argtypes = [P, ctypes.c_int32, P, ctypes.c_int64]
     L.scadyn_op_matvec.argtypes = [P, ctypes.c_int32, dp, dp]
+    L.scadyn_op_matvec_prec.argtypes = [P, ctypes.c_int32, ctypes.c_int32, dp, dp]
     L.scadyn_op_solve.argtypes = [P, ctypes.c_int32, dp, dp, ip]
     L.scadyn_op_vswf.argtypes = [P, ctypes.c_int32, dp]
     L.scadyn_op_tmatrix.argtypes = [P, ctypes.c_int32, dp, dp, dp, dp, ip]
@@ -174,10 +175,11 @@ class Operator:
             a = np.ascontiguousarray(a.transpose(0, 2, 1))  # -> [p][i][j]
         return a
 
-    def matvec(self, x):
+    def matvec(self, x, precision=PREC_FP64):
         x = np.ascontiguousarray(np.atleast_2d(x), dtype=np.complex128)
         y = np.zeros_like(x)
-        check(lib().scadyn_op_matvec(self.h, x.shape[0], _dp(x.view(np.float64)), _dp(y.view(np.float64))))
+        check(lib().scadyn_op_matvec_prec(self.h, precision, x.shape[0], _dp(x.view(np.float64)),
+                                          _dp(y.view(np.float64))))
         return y
 
     def solve(self, rhs):

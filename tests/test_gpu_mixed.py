@@ -1,0 +1,60 @@
+"""Mixed precision (FP32 matvec inside FP64 iterative refinement) vs the FP64 path.
+
+Tolerances:
+  FP32 matvec vs FP64 matvec         2e-6 relative (complex64 S*, grids, FFTs)
+  mixed solve vs tight FP64 solve    1e-8 relative (refine_tol 1e-10)
+  mixed T vs tight FP64 T            1e-6 relative (the parity bar)
+"""
+import numpy as np
+import pytest
+
+from helpers import rel, sphere_case
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f'[gpu-mixed] {msg}', flush=True)
+
+
+@pytest.fixture(scope='module')
+def case():
+    c, e, param, k = sphere_case(n_cells=6)
+    return c, e, param, k
+
+
+def test_fp32_matvec(case):
+    from scadyn_amd import native
+    c, e, param, k = case
+    say(f'operator {e.shape[0]} tets')
+    op = native.Operator(c, e, param, k)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((4, 3 * e.shape[0])) + 1j * rng.standard_normal((4, 3 * e.shape[0]))
+    y64 = op.matvec(x)
+    y32 = op.matvec(x, precision=native.PREC_MIXED)
+    for r in range(4):
+        assert rel(y32[r], y64[r]) < 2e-6
+    op.close()
+
+
+def test_mixed_solve_and_tmatrix(case):
+    from scadyn_amd import native
+    c, e, param, k = case
+    say('fp64 tight')
+    op64 = native.Operator(c, e, param, k, tol=1e-11, restart=20)
+    say('mixed')
+    opm = native.Operator(c, e, param, k, precision=native.PREC_MIXED, tol=1e-5, refine_tol=1e-10)
+    rng = np.random.default_rng(6)
+    b = rng.standard_normal((3, 3 * e.shape[0])) + 1j * rng.standard_normal((3, 3 * e.shape[0]))
+    x64, _ = op64.solve(b)
+    xm, _ = opm.solve(b)
+    for r in range(3):
+        assert rel(xm[r], x64[r]) < 1e-8
+    T64, _ = op64.tmatrix(3)
+    Tm, _ = opm.tmatrix(3)
+    num = np.sqrt(sum(np.linalg.norm(a - b) ** 2 for a, b in zip(Tm, T64)))
+    den = np.sqrt(sum(np.linalg.norm(b) ** 2 for b in T64))
+    say(f'T relative difference {num / den:.2e}')
+    assert num / den < 1e-6
+    op64.close()
+    opm.close()
