@@ -157,6 +157,45 @@ int scadyn_op_tmatrix(scadyn_operator *op, int32_t nmax,
 int scadyn_op_profile(scadyn_operator *op, int32_t enable);
 int scadyn_op_timings(scadyn_operator *op, double *timings /* [COUNT][3] */);
 
+/* ---- request loop (serving) ------------------------------------------ */
+
+/* Device bytes one T-matrix request needs at its peak (operator + the larger of
+ * setup temporaries and solve workspace); host-side computation only.  No
+ * counterpart in the reference (it has no server); the admission model of the
+ * request loop.  The reference prints a similar estimate
+ * (src/precorrection.f90:166-178). */
+int scadyn_footprint(const scadyn_mesh *mesh, const scadyn_opts *opts, int32_t nmax, int64_t *bytes);
+
+typedef struct scadyn_server scadyn_server;
+typedef struct scadyn_server_opts {
+    int32_t device;       /* HIP device ordinal                                   */
+    int32_t max_active;   /* requests resident on the device at once (0: 8)       */
+    int64_t hbm_budget;   /* device bytes the server may hold (0: 80% of free)    */
+    int32_t reserved[8];
+} scadyn_server_opts;
+
+#define SCADYN_REQ_QUEUED  0
+#define SCADYN_REQ_RUNNING 1
+#define SCADYN_REQ_DONE    2
+#define SCADYN_REQ_FAILED  3
+
+/* A request loop on one device: T-matrix requests (mesh, k, nmax) are queued,
+ * admitted FIFO while their footprint fits the budget, set up and solved with
+ * their steps interleaved on separate streams (continuous batching).  No
+ * counterpart in the reference. */
+int scadyn_server_create(const scadyn_server_opts *opts, scadyn_server **out);
+/* Copies the inputs; validates and sizes the request on the host (a request
+ * larger than the whole budget fails with SCADYN_ERR_MEMORY); returns its id. */
+int scadyn_server_submit(scadyn_server *sv, const scadyn_mesh *mesh, const scadyn_opts *opts, double k,
+                         int32_t nmax, int64_t *id);
+/* state and times {submit, admit, solve start, done} in seconds since creation */
+int scadyn_server_status(scadyn_server *sv, int64_t id, int32_t *state, double times[4]);
+/* Blocks until the request ends; copies T blocks (as scadyn_tmat_solve), the
+ * iteration counts and the times; releases the request. Returns its error. */
+int scadyn_server_wait(scadyn_server *sv, int64_t id, double *Taa, double *Tab, double *Tba, double *Tbb,
+                       int32_t *iters, double times[4]);
+void scadyn_server_destroy(scadyn_server *sv);
+
 #ifdef __cplusplus
 }
 #endif

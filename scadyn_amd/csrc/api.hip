@@ -3,7 +3,12 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <condition_variable>
+#include <deque>
+#include <map>
 #include <memory>
+#include <mutex>
+#include <thread>
 
 #include "fft.hpp"
 #include "internal.hpp"
@@ -125,7 +130,7 @@ struct Workspace {
     double *nrm2 = nullptr, *nav2 = nullptr, *pnrm = nullptr, *pnav = nullptr;
     cd *pdot = nullptr;
     long long *in_off = nullptr;
-    int *col = nullptr, *active = nullptr;
+    int *col = nullptr, *active = nullptr, *ret = nullptr;
     int nwg = 0;
     LinePass pass[2][5];  // [fp64, fp32][x fwd, y fwd, z conv, y inv, x inv]
 };
@@ -383,6 +388,7 @@ static Workspace &workspace(scadyn_operator *op, int B) {
     w.pdot = w.mem.alloc<cd>((size_t)w.nwg * B);
     w.in_off = w.mem.alloc<long long>(B);
     w.col = w.mem.alloc<int>(B);
+    w.ret = w.mem.alloc<int>(B);
     w.active = w.mem.alloc<int>(B);
     const size_t budget = 64 * 1024;
     for (int pr = 0; pr < 2; ++pr) {
@@ -553,117 +559,174 @@ struct SolveSpec {
     int restart = 4;
 };
 
-// Solve A x = src(:, c) for every column c of `cols` (src, dst: n x ld, internal
-// rows); continuous batching: a slot is refilled as soon as its system finishes.
-static void solve_columns(scadyn_operator *op, const cd *src, long long ld, const std::vector<int> &cols, cd *dst,
-                          int32_t *iters, int *nconv_fail, const SolveSpec &spec) {
-    const int ncols = (int)cols.size();
-    if (ncols == 0) return;
-    const int prec = spec.prec;
-    const int B = auto_batch(op, ncols);
-    Workspace &w = workspace(op, B);
-    hipStream_t s = op->s;
+// Pinned host buffer (page-locked, so async copies never read a reused buffer).
+template <class T>
+struct Pinned {
+    T *p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count) {
+        release();
+        HIP_CHECK(hipHostMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T), hipHostMallocDefault));
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+    }
+    ~Pinned() { release(); }
+};
+
+// Batched GMRES over a list of columns, advanced one matvec per step.
+// launch() enqueues a step on the operator's stream (refill of free slots,
+// input selection, matvec, MGS passes, Givens/least squares, vector update,
+// copy of the slot states); finish() waits for that step and retires finished
+// slots.  One host synchronisation per step; several solvers on different
+// operators (streams) overlap on the device when their launches interleave.
+struct ColumnSolver {
+    scadyn_operator *op = nullptr;
+    SolveSpec spec;
+    const cd *src = nullptr;
+    long long ld = 0;
+    std::vector<int> cols;
+    cd *dst = nullptr;
+    int32_t *iters = nullptr;
+    Workspace *w = nullptr;
     KrylovDims d;
-    d.n = w.n;
-    d.B = B;
-    d.m = w.m;
-    d.nwg = w.nwg;
-    const size_t nB = (size_t)w.n * B;
-    std::vector<SlotState> hs(B);
-    std::vector<int> slot_col(B, -1), loadc(B, -1), act(B, 0);
-    std::vector<long long> off(B, 0);
+    int B = 0;
+    Pinned<SlotState> hs;
+    Pinned<int> h_load, h_ret;
+    std::vector<int> slot_col;
     int next = 0, nfail = 0, singular = 0;
-    HIP_CHECK(hipMemsetAsync(w.st, 0, sizeof(SlotState) * B, s));
-    HIP_CHECK(hipMemsetAsync(w.V, 0, sizeof(cd) * (size_t)(w.m + 2) * nB, s));
-    // every slot finishes within restart*(m+1) matvecs; bound the loop
-    const long long max_steps = ((long long)spec.restart * (w.m + 1) + 4) * ((ncols + B - 1) / B + 2) + 16;
-    long long steps = 0;
-    for (;;) {
+    long long steps = 0, max_steps = 0;
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+
+    ~ColumnSolver() {
+        if (ev) (void)hipEventDestroy(ev);
+    }
+
+    void init(scadyn_operator *o, const cd *s_, long long ld_, const std::vector<int> &c_, cd *dst_, int32_t *it_,
+              const SolveSpec &sp) {
+        op = o;
+        src = s_;
+        ld = ld_;
+        cols = c_;
+        dst = dst_;
+        iters = it_;
+        spec = sp;
+        B = auto_batch(op, (int)cols.size());
+        w = &workspace(op, B);
+        d.n = w->n;
+        d.B = B;
+        d.m = w->m;
+        d.nwg = w->nwg;
+        hs.alloc(B);
+        h_load.alloc(B);
+        h_ret.alloc(B);
+        std::memset(hs.p, 0, sizeof(SlotState) * B);
+        slot_col.assign(B, -1);
+        const size_t nB = (size_t)w->n * B;
+        HIP_CHECK(hipMemsetAsync(w->st, 0, sizeof(SlotState) * B, op->s));
+        HIP_CHECK(hipMemsetAsync(w->V, 0, sizeof(cd) * (size_t)(w->m + 2) * nB, op->s));
+        // every system finishes within restart*(m+1) matvecs; bound the loop
+        max_steps = ((long long)spec.restart * (w->m + 1) + 4) * (((long long)cols.size() + B - 1) / B + 2) + 16;
+        HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    }
+
+    bool done() const {
+        if (next < (int)cols.size()) return false;
+        for (int b = 0; b < B; ++b)
+            if (slot_col[b] >= 0) return false;
+        return true;
+    }
+
+    void launch() {
         if (++steps > max_steps) fail(SCADYN_ERR_DEVICE, "GMRES driver made no progress");
+        hipStream_t s = op->s;
         // refill free slots
         bool any_load = false;
         for (int b = 0; b < B; ++b) {
-            loadc[b] = -1;
-            if (slot_col[b] < 0 && next < ncols) {
-                loadc[b] = cols[next++];
-                slot_col[b] = loadc[b];
+            h_load.p[b] = -1;
+            if (slot_col[b] < 0 && next < (int)cols.size()) {
+                h_load.p[b] = cols[next++];
+                slot_col[b] = h_load.p[b];
                 any_load = true;
+                hs.p[b].phase = PH_RESTART;  // host view until the state copy returns
+                hs.p[b].i = 0;
             }
         }
         if (any_load) {
-            h2d_sync(w.col, loadc.data(), sizeof(int) * B, s);
-            launch_load_rhs(d, w.col, src, ld, w.Rhs, w.V, w.pnrm, spec.x0zero ? 1 : 0, s);
-            launch_load_finish(d, w.col, w.st, w.pnrm, s);
+            HIP_CHECK(hipMemcpyAsync(w->col, h_load.p, sizeof(int) * B, hipMemcpyHostToDevice, s));
+            launch_load_rhs(d, w->col, src, ld, w->Rhs, w->V, w->pnrm, spec.x0zero ? 1 : 0, s);
+            launch_load_finish(d, w->col, w->st, w->pnrm, s);
         }
-        HIP_CHECK(hipMemcpyAsync(hs.data(), w.st, sizeof(SlotState) * B, hipMemcpyDeviceToHost, s));
-        HIP_CHECK(hipStreamSynchronize(s));
-        bool busy = false;
+        // MGS passes needed: max over busy slots of i+1 (host view of the previous step)
         int K = -1;
         for (int b = 0; b < B; ++b) {
-            act[b] = 0;
-            off[b] = 0;
             if (slot_col[b] < 0) continue;
-            if (hs[b].phase == PH_RESTART) {
-                act[b] = 1;
-                off[b] = (long long)(w.m + 1) * nB;
-                busy = true;
-                K = std::max(K, 0);
-            } else if (hs[b].phase == PH_ARNOLDI) {
-                act[b] = 1;
-                off[b] = (long long)hs[b].i * nB;
-                busy = true;
-                K = std::max(K, hs[b].i + 1);
-            }
+            if (hs.p[b].phase == PH_RESTART) K = std::max(K, 0);
+            if (hs.p[b].phase == PH_ARNOLDI) K = std::max(K, hs.p[b].i + 1);
         }
-        if (busy) {
-            h2d_sync(w.in_off, off.data(), sizeof(long long) * B, s);
-            h2d_sync(w.active, act.data(), sizeof(int) * B, s);
-            if (prec == SCADYN_PREC_MIXED)
-                run_matvec<cf>(op, w, w.V, w.active, w.W);
+        if (K >= 0) {
+            launch_set_inputs(d, w->st, w->in_off, w->active, s);
+            if (spec.prec == SCADYN_PREC_MIXED)
+                run_matvec<cf>(op, *w, w->V, w->active, w->W);
             else
-                run_matvec<cd>(op, w, w.V, w.active, w.W);
-            {
-                ProfScope ps(op, SCADYN_KCLASS_KRYLOV, 0.0);
-                for (int kk = 0; kk <= K; ++kk) {
-                    launch_mgs_pass(d, kk, w.W, w.V, w.Rhs, w.st, w.H, w.pdot, w.pnrm, w.pnav, s);
-                    launch_mgs_reduce(d, kk, w.st, w.H, w.pdot, w.pnrm, w.pnav, w.nrm2, w.nav2, s);
-                }
-                launch_slot_update(d, w.st, w.H, w.cs, w.sn, w.g, w.y, w.nrm2, w.nav2, spec.tol, spec.restart, s);
-                launch_vec_update(d, w.st, w.W, w.V, w.y, s);
+                run_matvec<cd>(op, *w, w->V, w->active, w->W);
+            ProfScope ps(op, SCADYN_KCLASS_KRYLOV, 0.0);
+            for (int kk = 0; kk <= K; ++kk) {
+                launch_mgs_pass(d, kk, w->W, w->V, w->Rhs, w->st, w->H, w->pdot, w->pnrm, w->pnav, s);
+                launch_mgs_reduce(d, kk, w->st, w->H, w->pdot, w->pnrm, w->pnav, w->nrm2, w->nav2, s);
             }
-            HIP_CHECK(hipMemcpyAsync(hs.data(), w.st, sizeof(SlotState) * B, hipMemcpyDeviceToHost, s));
-            HIP_CHECK(hipStreamSynchronize(s));
-            if (op->prof.on) op->prof.collect();
+            launch_slot_update(d, w->st, w->H, w->cs, w->sn, w->g, w->y, w->nrm2, w->nav2, spec.tol, spec.restart,
+                               s);
+            launch_vec_update(d, w->st, w->W, w->V, w->y, s);
         }
-        // retire finished slots
+        HIP_CHECK(hipMemcpyAsync(hs.p, w->st, sizeof(SlotState) * B, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipEventRecord(ev, s));
+        pending = true;
+    }
+
+    void finish() {
+        if (!pending) return;
+        HIP_CHECK(hipEventSynchronize(ev));
+        pending = false;
+        if (op->prof.on) op->prof.collect();
         bool any_done = false;
         for (int b = 0; b < B; ++b) {
-            loadc[b] = -1;
-            if (slot_col[b] >= 0 && hs[b].phase == PH_DONE) {
-                loadc[b] = slot_col[b];
-                if (iters) iters[slot_col[b]] += hs[b].ite;
-                if (hs[b].conv == 0) nfail++;
-                if (hs[b].conv < 0) singular++;
+            h_ret.p[b] = -1;
+            if (slot_col[b] >= 0 && hs.p[b].phase == PH_DONE) {
+                h_ret.p[b] = slot_col[b];
+                if (iters) iters[slot_col[b]] += hs.p[b].ite;
+                if (hs.p[b].conv == 0) nfail++;
+                if (hs.p[b].conv < 0) singular++;
                 slot_col[b] = -1;
+                hs.p[b].phase = PH_IDLE;
                 any_done = true;
             }
         }
         if (any_done) {
-            h2d_sync(w.col, loadc.data(), sizeof(int) * B, s);
-            launch_store_x(d, w.col, w.V, dst, ld, s);
-            // mark retired slots idle
-            for (int b = 0; b < B; ++b)
-                if (loadc[b] >= 0) hs[b].phase = PH_IDLE;
-            h2d_sync(w.st, hs.data(), sizeof(SlotState) * B, s);
+            HIP_CHECK(hipMemcpyAsync(w->ret, h_ret.p, sizeof(int) * B, hipMemcpyHostToDevice, op->s));
+            launch_store_x(d, w->ret, w->V, dst, ld, op->s);
+            launch_retire(d, w->ret, w->st, op->s);
         }
-        bool all_idle = true;
-        for (int b = 0; b < B; ++b)
-            if (slot_col[b] >= 0) all_idle = false;
-        if (all_idle && next >= ncols) break;
     }
-    HIP_CHECK(hipStreamSynchronize(s));
-    if (singular) fail(SCADYN_ERR_SINGULAR, "Matrix is numerically singular! (GMRES least squares)");
-    if (nconv_fail) *nconv_fail += nfail;
+};
+
+// Solve A x = src(:, c) for every column c of `cols` (src, dst: n x ld, internal
+// rows); continuous batching: a slot is refilled as soon as its system finishes.
+static void solve_columns(scadyn_operator *op, const cd *src, long long ld, const std::vector<int> &cols, cd *dst,
+                          int32_t *iters, int *nconv_fail, const SolveSpec &spec) {
+    if (cols.empty()) return;
+    ColumnSolver cs;
+    cs.init(op, src, ld, cols, dst, iters, spec);
+    while (!cs.done()) {
+        cs.launch();
+        cs.finish();
+    }
+    HIP_CHECK(hipStreamSynchronize(op->s));
+    if (cs.singular) fail(SCADYN_ERR_SINGULAR, "Matrix is numerically singular! (GMRES least squares)");
+    if (nconv_fail) *nconv_fail += cs.nfail;
 }
 
 // ---- column utilities for the refinement loop (n x ld row-major, internal rows)
@@ -786,42 +849,315 @@ static void solve_all(scadyn_operator *op, const cd *Bm, long long n, int ld, cd
     if (outer_rounds) *outer_rounds = rounds;
 }
 
+// One T-matrix solve (compute_T_matrix, src/T_matrix.f90:240-277) as a
+// resumable job: launch()/finish() advance its column solver by one step, so a
+// host loop can interleave the steps of several jobs on different streams.
+// FP64: one batched GMRES over the 2nm columns.  Mixed: rounds of FP32-matvec
+// GMRES on the residuals with FP64 residual updates between rounds.
+struct TmatJob {
+    scadyn_operator *op = nullptr;
+    int nmax = 0, nm = 0, ncol = 0;
+    long long n = 0;
+    bool mixed = false;
+    DevMem tmp;
+    cd *mat = nullptr, *X = nullptr, *R = nullptr, *D = nullptr;
+    int *mask = nullptr;
+    std::vector<double> bn;
+    std::vector<int> active;
+    std::vector<int32_t> iters;
+    int rounds = 0, nfail = 0;
+    bool finished = false;
+    std::unique_ptr<ColumnSolver> cs;
+    SolveSpec spec;
+
+    void start(scadyn_operator *o, int nmax_) {
+        op = o;
+        nmax = nmax_;
+        if (nmax < 1) fail(SCADYN_ERR_ARG, "nmax must be >= 1");
+        nm = (nmax + 1) * (nmax + 1) - 1;
+        ncol = 2 * nm;
+        n = 3LL * op->g.N;
+        mixed = op->opts.precision == SCADYN_PREC_MIXED;
+        hipStream_t s = op->s;
+        mat = tmp.alloc<cd>((size_t)n * ncol);
+        launch_vswf(op->d_tcoord, op->d_vol, op->g.N, nmax, op->k, mat, ncol, s);
+        X = tmp.alloc<cd>((size_t)n * ncol);
+        iters.assign(ncol, 0);
+        active.resize(ncol);
+        for (int c = 0; c < ncol; ++c) active[c] = c;
+        spec.tol = op->opts.tol;
+        spec.restart = op->opts.restart;
+        cs.reset(new ColumnSolver());
+        if (!mixed) {
+            cs->init(op, mat, ncol, active, X, iters.data(), spec);
+            return;
+        }
+        spec.prec = SCADYN_PREC_MIXED;
+        spec.x0zero = true;
+        R = tmp.alloc<cd>((size_t)n * ncol);
+        D = tmp.alloc<cd>((size_t)n * ncol);
+        mask = tmp.alloc<int>(ncol);
+        HIP_CHECK(hipMemsetAsync(X, 0, sizeof(cd) * (size_t)n * ncol, s));
+        HIP_CHECK(hipMemcpyAsync(R, mat, sizeof(cd) * (size_t)n * ncol, hipMemcpyDeviceToDevice, s));
+        bn = col_norms(op, mat, n, ncol);
+        cs->init(op, R, ncol, active, D, iters.data(), spec);
+    }
+
+    void launch() {
+        if (!finished) cs->launch();
+    }
+
+    void finish() {
+        if (finished) return;
+        cs->finish();
+        if (!cs->done()) return;
+        HIP_CHECK(hipStreamSynchronize(op->s));
+        if (cs->singular) fail(SCADYN_ERR_SINGULAR, "Matrix is numerically singular! (GMRES least squares)");
+        ++rounds;
+        if (!mixed) {
+            nfail += cs->nfail;
+            finished = true;
+            return;
+        }
+        std::vector<int> m(ncol, 0);
+        for (int c : active) m[c] = 1;
+        h2d_sync(mask, m.data(), sizeof(int) * ncol, op->s);
+        hipLaunchKernelGGL(k_add_cols, dim3(nblk(n * ncol)), dim3(256), 0, op->s, X, D, n, ncol, mask);
+        fp64_residual(op, mat, X, n, ncol, active, R);
+        std::vector<double> rn = col_norms(op, R, n, ncol);
+        std::vector<int> still;
+        for (int c : active)
+            if (!(rn[c] <= op->opts.refine_tol * bn[c])) still.push_back(c);
+        active.swap(still);
+        if (active.empty() || rounds >= 30) {
+            nfail += (int)active.size();
+            finished = true;
+            return;
+        }
+        cs.reset(new ColumnSolver());
+        cs->init(op, R, ncol, active, D, iters.data(), spec);
+    }
+
+    // T = i k^3 mat^H X split into the four nm x nm column-major blocks
+    void result(double *Taa, double *Tab, double *Tba, double *Tbb, int32_t *it_out) {
+        hipStream_t s = op->s;
+        const size_t work = std::max<size_t>((size_t)ncol * ncol * 64, 1 << 20);
+        DevMem t2;
+        cd *wk = t2.alloc<cd>(work);
+        cd *T = t2.alloc<cd>((size_t)ncol * ncol);
+        launch_tgemm(mat, X, n, ncol, T, wk, work, s);
+        std::vector<cd> hT((size_t)ncol * ncol);
+        HIP_CHECK(hipMemcpyAsync(hT.data(), T, sizeof(cd) * hT.size(), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        // sc = i k^3; Taa = T(1:nm,1:nm)*sc etc. (src/T_matrix.f90:270-275)
+        const double k3 = std::pow(op->k, 3.0);
+        double *outs[4] = {Taa, Tab, Tba, Tbb};
+        for (int q = 0; q < 4; ++q) {
+            const int r0 = (q >= 2) ? nm : 0, c0 = (q % 2) ? nm : 0;
+            for (int j = 0; j < nm; ++j)
+                for (int i = 0; i < nm; ++i) {
+                    cd v = hT[(size_t)(r0 + i) * ncol + (c0 + j)];
+                    // (v.x + i v.y) * (0 + i k3) = (-v.y k3) + i (v.x k3)
+                    outs[q][2 * ((size_t)j * nm + i)] = -v.y * k3;
+                    outs[q][2 * ((size_t)j * nm + i) + 1] = v.x * k3;
+                }
+        }
+        if (it_out) std::copy(iters.begin(), iters.end(), it_out);
+        op->last_outer_rounds = rounds;
+    }
+};
+
 static void tmatrix(scadyn_operator *op, int nmax, double *Taa, double *Tab, double *Tba, double *Tbb,
                     int32_t *iters) {
-    if (nmax < 1) fail(SCADYN_ERR_ARG, "nmax must be >= 1");
-    const int nm = (nmax + 1) * (nmax + 1) - 1, ncol = 2 * nm;
-    const long long n = 3LL * op->g.N;
-    hipStream_t s = op->s;
-    DevMem tmp;
-    cd *mat = tmp.alloc<cd>((size_t)n * ncol);
-    launch_vswf(op->d_tcoord, op->d_vol, op->g.N, nmax, op->k, mat, ncol, s);
-    cd *X = tmp.alloc<cd>((size_t)n * ncol);
-    int nfail = 0;
-    solve_all(op, mat, n, ncol, X, iters, &nfail, &op->last_outer_rounds);
-    const size_t work = std::max<size_t>((size_t)ncol * ncol * 64, 1 << 20);
-    cd *wk = tmp.alloc<cd>(work);
-    cd *T = tmp.alloc<cd>((size_t)ncol * ncol);
-    launch_tgemm(mat, X, n, ncol, T, wk, work, s);
-    std::vector<cd> hT((size_t)ncol * ncol);
-    HIP_CHECK(hipMemcpyAsync(hT.data(), T, sizeof(cd) * hT.size(), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-    // sc = i k^3; Taa = T(1:nm,1:nm)*sc etc., column-major outputs (src/T_matrix.f90:270-275)
-    const double k3 = std::pow(op->k, 3.0);
-    double *outs[4] = {Taa, Tab, Tba, Tbb};
-    for (int q = 0; q < 4; ++q) {
-        const int r0 = (q >= 2) ? nm : 0, c0 = (q % 2) ? nm : 0;
-        for (int j = 0; j < nm; ++j)
-            for (int i = 0; i < nm; ++i) {
-                cd v = hT[(size_t)(r0 + i) * ncol + (c0 + j)];
-                // (v.x + i v.y) * (0 + i k3) = (-v.y k3) + i (v.x k3)
-                outs[q][2 * ((size_t)j * nm + i)] = -v.y * k3;
-                outs[q][2 * ((size_t)j * nm + i) + 1] = v.x * k3;
-            }
+    TmatJob job;
+    job.start(op, nmax);
+    while (!job.finished) {
+        job.launch();
+        job.finish();
     }
-    if (nfail && op->opts.strict)
-        fail(SCADYN_ERR_NOCONV, std::to_string(nfail) + " T-matrix columns did not reach tol");
+    job.result(Taa, Tab, Tba, Tbb, iters);
+    if (job.nfail && op->opts.strict)
+        fail(SCADYN_ERR_NOCONV, std::to_string(job.nfail) + " T-matrix columns did not reach tol");
 }
 
+// ---------------------------------------------------------------- footprint model
+// Device bytes one T-matrix request needs at its peak: operator + the larger of
+// (setup temporaries) and (solve workspace + right-hand sides/solutions).
+struct Footprint {
+    size_t op = 0, setup = 0, solve = 0;
+    size_t peak() const { return op + std::max(setup, solve); }
+};
+
+static Footprint footprint_of(const HostGeometry &g, long long P, const scadyn_opts &o, int nmax, int B) {
+    Footprint f;
+    const size_t N = g.N, n = 3 * N;
+    const int Nax[3] = {g.Nx, g.Ny, g.Nz};
+    size_t L[3], H[3];
+    for (int c = 0; c < 3; ++c) {
+        L[c] = o.fft_pad == 1 ? 2 * Nax[c] : smooth_length(2 * Nax[c] - 1);
+        H[c] = L[c] / 2 + 1;
+    }
+    const size_t cdb = sizeof(cd);
+    f.op = N * (96 + 8 + 16 + 4 + 4) + 4 * (g.N_cubes + 1) + 1728 * N + 72 * (size_t)P + 8 * (N + 1) +
+           8 * 2 * (size_t)P + cdb * H[0] * H[1] * H[2] + cdb * (L[0] + L[1] + L[2]);
+    if (o.precision == SCADYN_PREC_MIXED) f.op += 864 * N + 8 * H[0] * H[1] * H[2];
+    f.setup = 2 * cdb * L[0] * L[1] * L[2] + 9 * (size_t)P + cdb * 27 * 27 * 27 + cdb * 98 * 27;
+    const int m = o.maxit;
+    const size_t grid = L[0] * L[1] * (size_t)g.Nz * 4;
+    const size_t ws = (size_t)B * cdb * ((size_t)(m + 4) * n + 2 * grid) + (size_t)B * 1024 * (m + 8);
+    const size_t ncol = 2 * ((size_t)(nmax + 1) * (nmax + 1) - 1);
+    size_t rhs = 2 * cdb * n * ncol + cdb * ncol * ncol * 65;
+    if (o.precision == SCADYN_PREC_MIXED) rhs += 2 * cdb * n * ncol;
+    f.solve = ws + rhs;
+    return f;
+}
+
+// ---------------------------------------------------------------- request loop
+enum ReqState : int { REQ_QUEUED = 0, REQ_RUNNING = 1, REQ_DONE = 2, REQ_FAILED = 3 };
+
+struct Request {
+    int64_t id = 0;
+    std::vector<double> coord, param;
+    std::vector<int32_t> etopol;
+    int32_t n_node = 0, n_tet = 0;
+    scadyn_opts opts;
+    double k = 0;
+    int nmax = 0;
+    size_t fp = 0;
+    int state = REQ_QUEUED;
+    bool set_up = false;
+    int err = 0;
+    std::string msg;
+    double t[4] = {0, 0, 0, 0};  // submit, admit, solve start, done (s since server start)
+    std::unique_ptr<scadyn_operator> op;
+    std::unique_ptr<TmatJob> job;
+    std::vector<double> T;  // 4 blocks, column-major complex
+    std::vector<int32_t> iters;
+};
+
+}  // namespace scadyn
+
+struct scadyn_server {
+    int device = 0, max_active = 8;
+    size_t budget = 0, in_use = 0;
+    std::mutex mu;
+    std::condition_variable cv, done_cv;
+    std::deque<scadyn::Request *> queue;
+    std::vector<scadyn::Request *> active;
+    std::map<int64_t, std::unique_ptr<scadyn::Request>> all;
+    int64_t next_id = 1;
+    bool stop = false;
+    std::thread worker;
+    std::chrono::steady_clock::time_point t0;
+    double now() const {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    void run();
+};
+
+namespace scadyn {
+
+static void fail_request(scadyn_server *sv, Request *r, int code, const std::string &msg) {
+    r->err = code;
+    r->msg = msg;
+    r->state = REQ_FAILED;
+    r->t[3] = sv->now();
+    r->job.reset();
+    r->op.reset();
+}
+
+}  // namespace scadyn
+
+void scadyn_server::run() {
+    using namespace scadyn;
+    (void)hipSetDevice(device);
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            // admission by device-memory footprint, FIFO
+            while (!queue.empty()) {
+                Request *r = queue.front();
+                if ((int)active.size() >= max_active || in_use + r->fp > budget) break;
+                queue.pop_front();
+                r->state = REQ_RUNNING;
+                r->t[1] = now();
+                in_use += r->fp;
+                active.push_back(r);
+            }
+            if (active.empty()) {
+                if (stop && queue.empty()) return;
+                cv.wait_for(lk, std::chrono::milliseconds(20));
+                continue;
+            }
+        }
+        // set up at most one admitted request per pass (host geometry + device setup)
+        for (Request *r : active) {
+            if (r->set_up || r->state != REQ_RUNNING) continue;
+            try {
+                scadyn_mesh m{r->n_node, r->n_tet, r->coord.data(), r->etopol.data(), r->param.data()};
+                r->op.reset(new scadyn_operator());
+                create_operator(r->op.get(), m, r->opts, r->k);
+                r->job.reset(new TmatJob());
+                r->job->start(r->op.get(), r->nmax);
+                r->set_up = true;
+                r->t[2] = now();
+            } catch (const Error &e) {
+                fail_request(this, r, e.code, e.msg);
+            }
+            break;
+        }
+        // one step of every solving request: launch all, then complete all
+        for (Request *r : active)
+            if (r->set_up && r->state == REQ_RUNNING) {
+                try {
+                    r->job->launch();
+                } catch (const Error &e) {
+                    fail_request(this, r, e.code, e.msg);
+                }
+            }
+        for (Request *r : active)
+            if (r->set_up && r->state == REQ_RUNNING) {
+                try {
+                    r->job->finish();
+                    if (r->job->finished) {
+                        const int nm = (r->nmax + 1) * (r->nmax + 1) - 1;
+                        r->T.assign((size_t)8 * nm * nm, 0.0);
+                        r->iters.assign(2 * nm, 0);
+                        const size_t blk = (size_t)2 * nm * nm;
+                        r->job->result(r->T.data(), r->T.data() + blk, r->T.data() + 2 * blk,
+                                       r->T.data() + 3 * blk, r->iters.data());
+                        const int nf = r->job->nfail;
+                        r->job.reset();
+                        r->op.reset();
+                        r->t[3] = now();
+                        if (nf && r->opts.strict) {
+                            r->err = SCADYN_ERR_NOCONV;
+                            r->msg = std::to_string(nf) + " T-matrix columns did not reach tol";
+                            r->state = REQ_FAILED;
+                        } else {
+                            r->state = REQ_DONE;
+                        }
+                    }
+                } catch (const Error &e) {
+                    fail_request(this, r, e.code, e.msg);
+                }
+            }
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            std::vector<Request *> keep;
+            for (Request *r : active) {
+                if (r->state == REQ_DONE || r->state == REQ_FAILED)
+                    in_use -= r->fp;
+                else
+                    keep.push_back(r);
+            }
+            if (keep.size() != active.size()) done_cv.notify_all();
+            active.swap(keep);
+        }
+    }
+}
+
+namespace scadyn {
 }  // namespace scadyn
 
 // ================================================================ C ABI
@@ -1059,6 +1395,145 @@ int scadyn_op_timings(scadyn_operator *op, double *t) {
         t[3 * i + 2] = op->prof.byt[i];
     }
     ABI_CATCH
+}
+
+int scadyn_footprint(const scadyn_mesh *mesh, const scadyn_opts *opts, int32_t nmax, int64_t *bytes) {
+    ABI_TRY
+    if (!mesh || !bytes) fail(SCADYN_ERR_ARG, "null argument");
+    scadyn_opts o;
+    if (opts)
+        o = *opts;
+    else
+        scadyn_opts_default(&o);
+    HostGeometry g;
+    build_geometry(*mesh, o, g);
+    NearPairs np;
+    build_near_pairs(g, np);
+    const int ncol = 2 * ((nmax + 1) * (nmax + 1) - 1);
+    const int B = o.batch > 0 ? std::min(o.batch, ncol) : std::min(ncol, 64);
+    *bytes = (int64_t)footprint_of(g, (long long)np.T.size(), o, nmax, B).peak();
+    ABI_CATCH
+}
+
+int scadyn_server_create(const scadyn_server_opts *so, scadyn_server **out) {
+    ABI_TRY
+    if (!out) fail(SCADYN_ERR_ARG, "null argument");
+    std::unique_ptr<scadyn_server> sv(new scadyn_server());
+    sv->device = so ? so->device : 0;
+    sv->max_active = (so && so->max_active > 0) ? so->max_active : 8;
+    size_t budget = so ? (size_t)std::max<int64_t>(so->hbm_budget, 0) : 0;
+    if (budget == 0) {
+        HIP_CHECK(hipSetDevice(sv->device));
+        size_t fr = 0, tot = 0;
+        HIP_CHECK(hipMemGetInfo(&fr, &tot));
+        budget = (size_t)(0.8 * (double)fr);
+    }
+    sv->budget = budget;
+    sv->t0 = std::chrono::steady_clock::now();
+    scadyn_server *p = sv.get();
+    sv->worker = std::thread([p] { p->run(); });
+    *out = sv.release();
+    ABI_CATCH
+}
+
+int scadyn_server_submit(scadyn_server *sv, const scadyn_mesh *mesh, const scadyn_opts *opts, double k, int32_t nmax,
+                         int64_t *id) {
+    ABI_TRY
+    if (!sv || !mesh || !id) fail(SCADYN_ERR_ARG, "null argument");
+    if (nmax < 1) fail(SCADYN_ERR_ARG, "nmax must be >= 1");
+    if (!(k > 0.0)) fail(SCADYN_ERR_ARG, "wavenumber must be positive");
+    std::unique_ptr<Request> r(new Request());
+    if (opts)
+        r->opts = *opts;
+    else
+        scadyn_opts_default(&r->opts);
+    const int ncol = 2 * ((nmax + 1) * (nmax + 1) - 1);
+    // a fixed slot count makes a request's arithmetic independent of its neighbours
+    if (r->opts.batch <= 0) r->opts.batch = std::min(ncol, 32);
+    r->opts.device = sv->device;
+    // validate and size on the host, before anything reaches the device
+    HostGeometry g;
+    build_geometry(*mesh, r->opts, g);
+    NearPairs np;
+    build_near_pairs(g, np);
+    r->fp = footprint_of(g, (long long)np.T.size(), r->opts, nmax, std::min(r->opts.batch, ncol)).peak();
+    if (r->fp > sv->budget)
+        fail(SCADYN_ERR_MEMORY, "request needs " + std::to_string(r->fp) + " device bytes, budget is " +
+                                    std::to_string(sv->budget));
+    r->n_node = mesh->n_node;
+    r->n_tet = mesh->n_tet;
+    r->coord.assign(mesh->coord, mesh->coord + 3 * (size_t)mesh->n_node);
+    r->etopol.assign(mesh->etopol, mesh->etopol + 4 * (size_t)mesh->n_tet);
+    r->param.assign(mesh->param, mesh->param + 2 * (size_t)mesh->n_tet);
+    r->k = k;
+    r->nmax = nmax;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        r->id = sv->next_id++;
+        r->t[0] = sv->now();
+        *id = r->id;
+        sv->queue.push_back(r.get());
+        sv->all[r->id] = std::move(r);
+    }
+    sv->cv.notify_all();
+    ABI_CATCH
+}
+
+int scadyn_server_status(scadyn_server *sv, int64_t id, int32_t *state, double times[4]) {
+    ABI_TRY
+    if (!sv) fail(SCADYN_ERR_ARG, "null server");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    auto it = sv->all.find(id);
+    if (it == sv->all.end()) fail(SCADYN_ERR_ARG, "unknown request id");
+    if (state) *state = it->second->state;
+    if (times)
+        for (int i = 0; i < 4; ++i) times[i] = it->second->t[i];
+    ABI_CATCH
+}
+
+int scadyn_server_wait(scadyn_server *sv, int64_t id, double *Taa, double *Tab, double *Tba, double *Tbb,
+                       int32_t *iters, double times[4]) {
+    ABI_TRY
+    if (!sv) fail(SCADYN_ERR_ARG, "null server");
+    std::unique_ptr<Request> r;
+    {
+        std::unique_lock<std::mutex> lk(sv->mu);
+        auto it = sv->all.find(id);
+        if (it == sv->all.end()) fail(SCADYN_ERR_ARG, "unknown request id");
+        Request *p = it->second.get();
+        sv->done_cv.wait(lk, [p] { return p->state == REQ_DONE || p->state == REQ_FAILED; });
+        r = std::move(it->second);
+        sv->all.erase(it);
+    }
+    if (times)
+        for (int i = 0; i < 4; ++i) times[i] = r->t[i];
+    if (r->state == REQ_FAILED) fail(r->err, r->msg);
+    const int nm = (r->nmax + 1) * (r->nmax + 1) - 1;
+    const size_t blk = (size_t)2 * nm * nm;
+    double *outs[4] = {Taa, Tab, Tba, Tbb};
+    for (int q = 0; q < 4; ++q)
+        if (outs[q]) std::copy(r->T.begin() + q * blk, r->T.begin() + (q + 1) * blk, outs[q]);
+    if (iters) std::copy(r->iters.begin(), r->iters.end(), iters);
+    ABI_CATCH
+}
+
+void scadyn_server_destroy(scadyn_server *sv) {
+    if (!sv) return;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->stop = true;
+        // requests still queued are dropped
+        for (scadyn::Request *r : sv->queue) {
+            r->state = scadyn::REQ_FAILED;
+            r->err = SCADYN_ERR_ARG;
+            r->msg = "server destroyed";
+        }
+        sv->queue.clear();
+    }
+    sv->cv.notify_all();
+    if (sv->worker.joinable()) sv->worker.join();
+    sv->done_cv.notify_all();
+    delete sv;
 }
 
 }  // extern "C"

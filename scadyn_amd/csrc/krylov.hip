@@ -304,6 +304,35 @@ __global__ void k_store_x(KrylovDims d, const int *__restrict__ col, const cd *_
     }
 }
 
+// matvec input offsets and activity from the slot phases: a restarting slot
+// multiplies x (block m+1 of V), an Arnoldi slot its current basis vector
+__global__ void k_set_inputs(KrylovDims d, const SlotState *__restrict__ st, long long *__restrict__ in_off,
+                             int *__restrict__ active) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= d.B) return;
+    const long long nB = d.n * d.B;
+    const SlotState s = st[b];
+    long long off = 0;
+    int a = 0;
+    if (s.phase == PH_RESTART) {
+        off = (long long)(d.m + 1) * nB;
+        a = 1;
+    } else if (s.phase == PH_ARNOLDI) {
+        off = (long long)s.i * nB;
+        a = 1;
+    }
+    in_off[b] = off;
+    active[b] = a;
+}
+
+// slots whose column was stored become idle
+__global__ void k_retire(KrylovDims d, const int *__restrict__ col, SlotState *__restrict__ st) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= d.B || col[b] < 0) return;
+    st[b].phase = PH_IDLE;
+    st[b].act = ACT_NONE;
+}
+
 // ---- T = mat^H X, split over row chunks -----------------------------------------
 __global__ __launch_bounds__(256) void k_tgemm(const cd *__restrict__ A, const cd *__restrict__ X, long long n,
                                                int ncol, long long chunk, cd *__restrict__ part) {
@@ -381,6 +410,14 @@ void launch_store_x(const KrylovDims &d, const int *col, const cd *V, cd *dst, l
     long long tot = d.n * d.B;
     unsigned g = (unsigned)std::min<long long>((tot + 255) / 256, 1 << 16);
     hipLaunchKernelGGL(k_store_x, dim3(g), dim3(256), 0, s, d, col, V, dst, ld);
+    HIP_CHECK(hipGetLastError());
+}
+void launch_set_inputs(const KrylovDims &d, const SlotState *st, long long *in_off, int *active, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_inputs, dim3((d.B + 63) / 64), dim3(64), 0, s, d, st, in_off, active);
+    HIP_CHECK(hipGetLastError());
+}
+void launch_retire(const KrylovDims &d, const int *col, SlotState *st, hipStream_t s) {
+    hipLaunchKernelGGL(k_retire, dim3((d.B + 63) / 64), dim3(64), 0, s, d, col, st);
     HIP_CHECK(hipGetLastError());
 }
 void launch_tgemm(const cd *mat, const cd *X, long long n, int ncol, cd *T, cd *work, size_t work_elems,

@@ -51,6 +51,11 @@ void launch_load_finish(const KrylovDims &d, const int *col, SlotState *st, cons
 // store X of slots with col[b] >= 0 into column col[b] of dst (n x ld)
 void launch_store_x(const KrylovDims &d, const int *col, const cd *V, cd *dst, long long ld, hipStream_t s);
 
+// matvec input offsets / activity derived on the device from the slot phases
+void launch_set_inputs(const KrylovDims &d, const SlotState *st, long long *in_off, int *active, hipStream_t s);
+// mark slots with col[b] >= 0 idle
+void launch_retire(const KrylovDims &d, const int *col, SlotState *st, hipStream_t s);
+
 // T = mat^H X: mat, X are n x ncol row-major; T is ncol x ncol row-major (i, j)
 void launch_tgemm(const cd *mat, const cd *X, long long n, int ncol, cd *T, cd *work, size_t work_elems,
                   hipStream_t s);

@@ -21,7 +21,9 @@ KCLASS = ['scatter', 'fftx', 'ffty', 'fftz_conv', 'iffty', 'ifftx', 'gather_near
 EXPORTED = ['scadyn_opts_default', 'scadyn_last_error', 'scadyn_tmat_solve', 'scadyn_op_create',
             'scadyn_op_destroy', 'scadyn_op_info', 'scadyn_op_export', 'scadyn_op_matvec', 'scadyn_op_matvec_prec',
             'scadyn_op_solve', 'scadyn_op_vswf', 'scadyn_op_tmatrix', 'scadyn_op_profile',
-            'scadyn_op_timings']
+            'scadyn_op_timings', 'scadyn_footprint', 'scadyn_server_create', 'scadyn_server_submit',
+            'scadyn_server_status', 'scadyn_server_wait', 'scadyn_server_destroy']
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 
 
 class ScadynError(RuntimeError):
@@ -35,6 +37,11 @@ class CMesh(ctypes.Structure):
                 ('coord', ctypes.POINTER(ctypes.c_double)),
                 ('etopol', ctypes.POINTER(ctypes.c_int32)),
                 ('param', ctypes.POINTER(ctypes.c_double))]
+
+
+class CServerOpts(ctypes.Structure):
+    _fields_ = [('device', ctypes.c_int32), ('max_active', ctypes.c_int32), ('hbm_budget', ctypes.c_int64),
+                ('reserved', ctypes.c_int32 * 8)]
 
 
 class COpts(ctypes.Structure):
@@ -77,8 +84,17 @@ def lib():
     L.scadyn_op_tmatrix.argtypes = [P, ctypes.c_int32, dp, dp, dp, dp, ip]
     L.scadyn_op_profile.argtypes = [P, ctypes.c_int32]
     L.scadyn_op_timings.argtypes = [P, dp]
+    L.scadyn_footprint.argtypes = [ctypes.POINTER(CMesh), ctypes.POINTER(COpts), ctypes.c_int32,
+                                   ctypes.POINTER(ctypes.c_int64)]
+    L.scadyn_server_create.argtypes = [ctypes.POINTER(CServerOpts), ctypes.POINTER(P)]
+    L.scadyn_server_submit.argtypes = [P, ctypes.POINTER(CMesh), ctypes.POINTER(COpts), ctypes.c_double,
+                                       ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]
+    L.scadyn_server_status.argtypes = [P, ctypes.c_int64, ip, dp]
+    L.scadyn_server_wait.argtypes = [P, ctypes.c_int64, dp, dp, dp, dp, ip, dp]
+    L.scadyn_server_destroy.argtypes = [P]
+    L.scadyn_server_destroy.restype = None
     for name in EXPORTED:
-        if name not in ('scadyn_opts_default', 'scadyn_last_error', 'scadyn_op_destroy'):
+        if name not in ('scadyn_opts_default', 'scadyn_last_error', 'scadyn_op_destroy', 'scadyn_server_destroy'):
             getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -223,3 +239,58 @@ def tmat_solve(coord, etopol, param, k, nmax, **opts):
     check(lib().scadyn_tmat_solve(ctypes.byref(m.c), ctypes.byref(o), float(k), nmax,
                                   *[_dp(t.view(np.float64)) for t in T], _ip(it)))
     return tuple(t.T.copy() for t in T), it
+
+
+def footprint(coord, etopol, param, nmax, **opts):
+    """Peak device bytes of one T-matrix request (host-side estimate)."""
+    m = MeshArrays(coord, etopol, param)
+    o = default_opts(**opts)
+    b = ctypes.c_int64()
+    check(lib().scadyn_footprint(ctypes.byref(m.c), ctypes.byref(o), nmax, ctypes.byref(b)))
+    return b.value
+
+
+class Server:
+    """The library's request loop: submit T-matrix requests, wait for results."""
+
+    def __init__(self, device=0, max_active=8, hbm_budget=0):
+        so = CServerOpts(device, max_active, hbm_budget)
+        h = ctypes.c_void_p()
+        check(lib().scadyn_server_create(ctypes.byref(so), ctypes.byref(h)))
+        self.h = h
+        self.nmax = {}
+
+    def submit(self, coord, etopol, param, k, nmax, **opts):
+        m = MeshArrays(coord, etopol, param)
+        o = default_opts(**opts)
+        rid = ctypes.c_int64()
+        check(lib().scadyn_server_submit(self.h, ctypes.byref(m.c), ctypes.byref(o), float(k), nmax,
+                                         ctypes.byref(rid)))
+        self.nmax[rid.value] = nmax
+        return rid.value
+
+    def status(self, rid):
+        st = np.zeros(1, np.int32)
+        t = np.zeros(4)
+        check(lib().scadyn_server_status(self.h, rid, _ip(st), _dp(t)))
+        return int(st[0]), t
+
+    def wait(self, rid):
+        nmax = self.nmax.pop(rid)
+        nm = (nmax + 1) ** 2 - 1
+        T = [np.zeros((nm, nm), np.complex128) for _ in range(4)]
+        it = np.zeros(2 * nm, np.int32)
+        t = np.zeros(4)
+        check(lib().scadyn_server_wait(self.h, rid, *[_dp(x.view(np.float64)) for x in T], _ip(it), _dp(t)))
+        return tuple(x.T.copy() for x in T), it, t
+
+    def close(self):
+        if getattr(self, 'h', None):
+            lib().scadyn_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
