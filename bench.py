@@ -38,6 +38,13 @@ def make_workload(name):
         eps, ka, nmax = 3.0 + 0.1j, 8.0, 12
         desc = ('Gaussian random ellipsoid 1:0.8:0.6, sigma 0.1, corr 20 deg, l<=10, seed 0; '
                 'cube-split; eps 3.0+0.1i; ka 8; nmax 12; FP64')
+    elif name == 'layered824k':
+        c, e = meshgen.sphere(64, seed=0)
+        ka, nmax = 15.0, 20
+        c = meshgen.rescale_to_radius(c, e, a)
+        param = meshgen.layered_param(c, e, 0.6, 3.5 + 0.2j, 2.0 + 0.02j, a_scale=a)
+        desc = ('sphere 64^3 cells, 2-layer eps (3.5+0.2i for r<0.6a, 2.0+0.02i), ka 15, nmax 20')
+        return c, e, param, ka / a, nmax, desc
     elif name == 'sphere43k':
         c, e = meshgen.sphere(24, seed=0)
         eps, ka, nmax = 2.5 + 0.01j, 2.0, 5
@@ -47,6 +54,80 @@ def make_workload(name):
     c = meshgen.rescale_to_radius(c, e, a)
     param = np.full(e.shape[0], eps)
     return c, e, param, ka / a, nmax, desc
+
+
+def nmax_for(x):
+    """Expansion order of a wavelength bar: ceil(x + 4 x^(1/3) + 2)."""
+    import math
+    return int(math.ceil(x + 4.0 * x ** (1.0 / 3.0) + 2.0))
+
+
+def ellipsoid_mesh(n_tets_target, seed):
+    from scadyn_amd import meshgen
+    n_cells = max(8, int(round(56 * (n_tets_target / 198390.0) ** (1.0 / 3.0))))
+    c, e = meshgen.gaussian_ellipsoid(n_cells=n_cells, seed=seed)
+    return meshgen.rescale_to_radius(c, e, 1e-7), e
+
+
+def request_stream(name, rng):
+    """(mesh, param, k, nmax, arrival offset s) lists for the request-loop workloads."""
+    a = 1e-7
+    reqs = []
+    if name == 'bars16':
+        # one Gaussian ellipsoid (~200k tets) at 16 size parameters log-spaced 2..15
+        c, e = ellipsoid_mesh(198390, 0)
+        for x in np.exp(np.linspace(np.log(2.0), np.log(15.0), 16)):
+            reqs.append((c, e, np.full(e.shape[0], 3.0 + 0.1j), x / a, nmax_for(x), 0.0))
+    elif name.startswith('serving'):
+        # Poisson arrivals of random-ellipsoid requests; sizes log-uniform in [lo, hi] tets
+        lo, hi, nreq, xmax = (20000, 400000, 24, 15.0) if name == 'serving' else (5000, 40000, 16, 6.0)
+        t = 0.0
+        for i in range(nreq):
+            ntet = float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+            x = float(rng.uniform(2.0, xmax))
+            eps = complex(rng.uniform(1.5, 4.0), rng.uniform(0.0, 0.3))
+            c, e = ellipsoid_mesh(ntet, seed=100 + i)
+            reqs.append((c, e, np.full(e.shape[0], eps), x / a, nmax_for(x), t))
+    return reqs
+
+
+def run_requests(name, args):
+    """Request-loop workloads: throughput and per-request latency."""
+    from scadyn_amd import native
+    rng = np.random.default_rng(0)
+    reqs = request_stream(name, rng)
+    prec = native.PREC_MIXED if args.precision == 'mixed' else native.PREC_FP64
+    cols = [2 * ((r[4] + 1) ** 2 - 1) for r in reqs]
+    if name.startswith('serving'):
+        # arrival rate for ~80% utilisation from a calibration solve of the first request
+        c, e, p, k, nmax, _ = reqs[0]
+        t = time.time()
+        native.tmat_solve(c, e, p, k, nmax, precision=prec, tol=args.tol, refine_tol=args.refine_tol)
+        work0 = cols[0] * e.shape[0]
+        rate_work = work0 / (time.time() - t)          # (columns x tets) per second
+        mean_service = np.mean([cc * r[1].shape[0] for cc, r in zip(cols, reqs)]) / rate_work
+        gaps = rng.exponential(mean_service / 0.8, len(reqs))
+        arrivals = np.concatenate([[0.0], np.cumsum(gaps[1:])])
+    else:
+        arrivals = np.zeros(len(reqs))
+    print(f'[bench] {name}: {len(reqs)} requests, {sum(cols)} columns', file=sys.stderr, flush=True)
+    sv = native.Server(max_active=args.max_active)
+    t0 = time.time()
+    ids = []
+    for (c, e, p, k, nmax, _), at in zip(reqs, arrivals):
+        while time.time() - t0 < at:
+            time.sleep(0.005)
+        ids.append(sv.submit(c, e, p, k, nmax, precision=prec, tol=args.tol, refine_tol=args.refine_tol))
+    lat = []
+    for rid in ids:
+        _, it, times = sv.wait(rid)
+        lat.append(times[3] - times[0])
+    makespan = time.time() - t0
+    sv.close()
+    lat = np.array(lat)
+    return {'columns': int(sum(cols)), 'requests': len(reqs), 'makespan_s': makespan,
+            'columns_per_s': sum(cols) / makespan, 'p50_latency_s': float(np.percentile(lat, 50)),
+            'p99_latency_s': float(np.percentile(lat, 99)), 'max_active': args.max_active}
 
 
 def cpu_baseline(op_gpu, c, e, param, k, matvecs_per_col, budget_s=25.0):
@@ -94,18 +175,29 @@ def main():
     ap.add_argument('--precision', default='fp64', choices=['fp64', 'mixed'])
     ap.add_argument('--tol', type=float, default=1e-5)
     ap.add_argument('--refine-tol', type=float, default=1e-9)
+    ap.add_argument('--max-active', type=int, default=8)
+    ap.add_argument('--ortho', type=int, default=0, help='0 MGS (reference), 1 CGS + DGKS')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU path is not built.',
               file=sys.stderr)
         sys.exit(2)
     from scadyn_amd import native
+    if args.workload in ('bars16', 'serving', 'serving-small'):
+        res = run_requests(args.workload, args)
+        out = {'metric': METRIC, 'value': res['columns_per_s'], 'unit': 'columns/s', 'n_gpus': 1,
+               'steps': 1, 'warmup': 0, 'ms_per_step': 1000.0 * res['makespan_s'], 'higher_is_better': True,
+               'scaling': 'weak', 'vs_baseline': None,
+               'dtype': 'f32 matvec + f64 refinement' if args.precision == 'mixed' else 'f64',
+               'data': 'synthetic (seeded mesh generator)', 'config': dict(workload=args.workload, **res)}
+        print(json.dumps(out), flush=True)
+        return
     c, e, param, k, nmax, desc = make_workload(args.workload)
     print(f'[bench] workload {args.workload}: {e.shape[0]} tets, nmax {nmax}', file=sys.stderr, flush=True)
     t0 = time.time()
     prec = native.PREC_MIXED if args.precision == 'mixed' else native.PREC_FP64
     op = native.Operator(c, e, param, k, batch=args.batch, precision=prec, tol=args.tol,
-                         refine_tol=args.refine_tol)
+                         refine_tol=args.refine_tol, ortho=args.ortho)
     setup_s = time.time() - t0
     info = op.info()
     print(f'[bench] setup {setup_s:.2f} s: {info}', file=sys.stderr, flush=True)

@@ -72,7 +72,11 @@ typedef struct scadyn_opts {
     int32_t fft_pad;     /* 0: smallest 2,3,5,7-smooth length >= 2N-1 per axis;
                             1: the reference's 2N                                       */
     double refine_tol;   /* mixed precision: outer FP64 relative residual target        */
-    int32_t reserved[8];
+    int32_t ortho;       /* Arnoldi orthogonalisation: 0 modified Gram-Schmidt (the
+                            reference's, src/gmres_module.f90:241-244); 1 classical
+                            Gram-Schmidt with DGKS re-orthogonalisation (two fused
+                            passes per iteration)                                       */
+    int32_t reserved[7];
 } scadyn_opts;
 
 /* Defaults of the reference (src/common.f90:78-89). */

@@ -130,7 +130,8 @@ struct Workspace {
     double *nrm2 = nullptr, *nav2 = nullptr, *pnrm = nullptr, *pnav = nullptr;
     cd *pdot = nullptr;
     long long *in_off = nullptr;
-    int *col = nullptr, *active = nullptr, *ret = nullptr;
+    int *col = nullptr, *active = nullptr, *ret = nullptr, *flag = nullptr;
+    cd *cc = nullptr, *hprev = nullptr;
     int nwg = 0;
     LinePass pass[2][5];  // [fp64, fp32][x fwd, y fwd, z conv, y inv, x inv]
 };
@@ -385,7 +386,10 @@ static Workspace &workspace(scadyn_operator *op, int B) {
     w.nav2 = w.mem.alloc<double>(B);
     w.pnrm = w.mem.alloc<double>((size_t)w.nwg * B);
     w.pnav = w.mem.alloc<double>((size_t)w.nwg * B);
-    w.pdot = w.mem.alloc<cd>((size_t)w.nwg * B);
+    w.pdot = w.mem.alloc<cd>((size_t)w.nwg * (m + 1) * B);
+    w.flag = w.mem.alloc<int>(B);
+    w.cc = w.mem.alloc<cd>((size_t)B * (m + 1));
+    w.hprev = w.mem.alloc<cd>((size_t)B * (m + 1));
     w.in_off = w.mem.alloc<long long>(B);
     w.col = w.mem.alloc<int>(B);
     w.ret = w.mem.alloc<int>(B);
@@ -557,6 +561,7 @@ struct SolveSpec {
     bool x0zero = false;          // x0 = 0 (correction equations) instead of x0 = rhs
     double tol = 1e-5;
     int restart = 4;
+    int ortho = 0;                // 0 MGS (reference), 1 CGS + DGKS
 };
 
 // Pinned host buffer (page-locked, so async copies never read a reused buffer).
@@ -674,9 +679,14 @@ struct ColumnSolver {
             else
                 run_matvec<cd>(op, *w, w->V, w->active, w->W);
             ProfScope ps(op, SCADYN_KCLASS_KRYLOV, 0.0);
-            for (int kk = 0; kk <= K; ++kk) {
-                launch_mgs_pass(d, kk, w->W, w->V, w->Rhs, w->st, w->H, w->pdot, w->pnrm, w->pnav, s);
-                launch_mgs_reduce(d, kk, w->st, w->H, w->pdot, w->pnrm, w->pnav, w->nrm2, w->nav2, s);
+            if (spec.ortho == 1) {
+                launch_cgs(d, w->W, w->V, w->Rhs, w->st, w->H, w->pdot, w->pnrm, w->pnav, w->nrm2, w->nav2, w->flag,
+                           w->cc, w->hprev, std::max(K - 1, 0), s);
+            } else {
+                for (int kk = 0; kk <= K; ++kk) {
+                    launch_mgs_pass(d, kk, w->W, w->V, w->Rhs, w->st, w->H, w->pdot, w->pnrm, w->pnav, s);
+                    launch_mgs_reduce(d, kk, w->st, w->H, w->pdot, w->pnrm, w->pnav, w->nrm2, w->nav2, s);
+                }
             }
             launch_slot_update(d, w->st, w->H, w->cs, w->sn, w->g, w->y, w->nrm2, w->nav2, spec.tol, spec.restart,
                                s);
@@ -814,6 +824,7 @@ static void solve_all(scadyn_operator *op, const cd *Bm, long long n, int ld, cd
     SolveSpec spec;
     spec.tol = op->opts.tol;
     spec.restart = op->opts.restart;
+    spec.ortho = op->opts.ortho;
     if (op->opts.precision != SCADYN_PREC_MIXED) {
         solve_columns(op, Bm, ld, all, X, iters, nfail, spec);
         if (outer_rounds) *outer_rounds = 1;
@@ -887,6 +898,7 @@ struct TmatJob {
         for (int c = 0; c < ncol; ++c) active[c] = c;
         spec.tol = op->opts.tol;
         spec.restart = op->opts.restart;
+        spec.ortho = op->opts.ortho;
         cs.reset(new ColumnSolver());
         if (!mixed) {
             cs->init(op, mat, ncol, active, X, iters.data(), spec);
@@ -1207,6 +1219,7 @@ int scadyn_op_create(const scadyn_mesh *mesh, const scadyn_opts *opts, double k,
     if (o.precision != SCADYN_PREC_FP64 && o.precision != SCADYN_PREC_MIXED)
         fail(SCADYN_ERR_ARG, "unknown precision mode");
     if (o.maxit < 1 || o.restart < 1) fail(SCADYN_ERR_ARG, "maxit and restart must be >= 1");
+    if (o.ortho != 0 && o.ortho != 1) fail(SCADYN_ERR_ARG, "ortho must be 0 (MGS) or 1 (CGS + DGKS)");
     *out = nullptr;
     std::unique_ptr<scadyn_operator> op(new scadyn_operator());
     create_operator(op.get(), *mesh, o, k);

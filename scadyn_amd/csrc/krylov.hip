@@ -116,6 +116,165 @@ __global__ void k_mgs_reduce(KrylovDims d, int k, const SlotState *__restrict__ 
     if (k == 0) nav2[b] = x2;
 }
 
+// ---- classical Gram-Schmidt with DGKS re-orthogonalisation (opts.ortho = 1) ----
+// Pass "dots": h_k = <v_k, w> for the 16 basis vectors k in [16 z, 16 z + 16) of
+// blockIdx.z (z = 0 also accumulates |w|^2 and turns restarting slots' W into
+// the residual rhs - A x).  Pass "update": w -= sum_k h_k v_k and |w|^2.  Pass 2
+// repeats both for slots whose norm dropped below 1/sqrt(2) of its value.
+constexpr int CGS_KC = 16;
+
+__global__ __launch_bounds__(256) void k_cgs_dots(KrylovDims d, int pass, cd *__restrict__ W,
+                                                  const cd *__restrict__ V, const cd *__restrict__ Rhs,
+                                                  const SlotState *__restrict__ st, const int *__restrict__ flag,
+                                                  cd *__restrict__ pdot, double *__restrict__ pnav) {
+    __shared__ cd sred[256];
+    __shared__ double snav[256];
+    RowSlotMap m = row_slot_map(d.B);
+    const int k0 = blockIdx.z * CGS_KC;
+    cd acc[CGS_KC];
+#pragma unroll
+    for (int q = 0; q < CGS_KC; ++q) acc[q] = cmk(0.0, 0.0);
+    double nav = 0.0;
+    int kmax = -1;  // highest basis index of this slot in this chunk
+    bool rst = false;
+    if (m.ok) {
+        const SlotState s = st[m.b];
+        const size_t nB = (size_t)d.n * d.B;
+        const bool arn = s.phase == PH_ARNOLDI && (pass == 0 || flag[m.b]);
+        rst = pass == 0 && s.phase == PH_RESTART && blockIdx.z == 0;
+        if (arn) kmax = min(s.i, k0 + CGS_KC - 1);
+        if (arn || rst)
+            for (long long row = (long long)blockIdx.x * m.RW + m.rl; row < d.n; row += (long long)gridDim.x * m.RW) {
+                const size_t e = (size_t)row * d.B + m.b;
+                cd w = W[e];
+                if (rst) {
+                    w = csub(Rhs[e], w);
+                    W[e] = w;
+                    nav += cabs2(w);
+                    continue;
+                }
+                if (blockIdx.z == 0) nav += cabs2(w);
+#pragma unroll
+                for (int q = 0; q < CGS_KC; ++q)
+                    if (k0 + q <= kmax) acc[q] = cadd(acc[q], cmulc(V[(size_t)(k0 + q) * nB + e], w));
+            }
+    }
+    snav[threadIdx.x] = nav;
+    __syncthreads();
+    if (m.ok && m.rl == 0 && blockIdx.z == 0) {
+        double a = 0;
+        for (int r = 0; r < m.RW; ++r) a += snav[r * m.SB + m.sl];
+        pnav[(size_t)blockIdx.x * d.B + m.b] = a;
+    }
+    const int nk = d.m + 1;
+    for (int q = 0; q < CGS_KC; ++q) {
+        if (k0 + q >= nk) break;
+        __syncthreads();
+        sred[threadIdx.x] = acc[q];
+        __syncthreads();
+        if (m.ok && m.rl == 0) {
+            cd a = cmk(0.0, 0.0);
+            for (int r = 0; r < m.RW; ++r) a = cadd(a, sred[r * m.SB + m.sl]);
+            pdot[((size_t)blockIdx.x * nk + (k0 + q)) * d.B + m.b] = a;
+        }
+    }
+}
+
+// sum partial dots into H column i (pass 0 stores, pass 1 adds); |w|^2 of restart slots
+__global__ void k_cgs_dots_reduce(KrylovDims d, int pass, const SlotState *__restrict__ st,
+                                  const int *__restrict__ flag, cd *__restrict__ Hall, const cd *__restrict__ pdot,
+                                  const double *__restrict__ pnav, double *__restrict__ nav2,
+                                  double *__restrict__ nrm2) {
+    const int nk = d.m + 1;
+    const int b = blockIdx.x;  // one block per slot, threads over k
+    if (b >= d.B) return;
+    const SlotState s = st[b];
+    if (threadIdx.x == 0 && pass == 0 && (s.phase == PH_ARNOLDI || s.phase == PH_RESTART)) {
+        double a = 0;
+        for (int w = 0; w < d.nwg; ++w) a += pnav[(size_t)w * d.B + b];
+        if (s.phase == PH_ARNOLDI)
+            nav2[b] = a;
+        else
+            nrm2[b] = a;
+    }
+    if (s.phase != PH_ARNOLDI || (pass == 1 && !flag[b])) return;
+    cd *H = Hall + (size_t)b * (d.m + 1) * d.m + (size_t)(d.m + 1) * s.i;
+    for (int k = threadIdx.x; k <= s.i; k += blockDim.x) {
+        cd a = cmk(0.0, 0.0);
+        for (int w = 0; w < d.nwg; ++w) a = cadd(a, pdot[((size_t)w * nk + k) * d.B + b]);
+        H[k] = pass == 0 ? a : cadd(H[k], a);
+    }
+}
+
+// w -= sum_{k<=i} c_k v_k with c = the dots of this pass (stored in H, or the
+// pass-2 increments kept in cc); partial |w|^2
+__global__ __launch_bounds__(256) void k_cgs_update(KrylovDims d, int pass, cd *__restrict__ W,
+                                                    const cd *__restrict__ V, const SlotState *__restrict__ st,
+                                                    const int *__restrict__ flag, const cd *__restrict__ cc,
+                                                    double *__restrict__ pnrm) {
+    __shared__ double snrm[256];
+    RowSlotMap m = row_slot_map(d.B);
+    double nrm = 0.0;
+    if (m.ok) {
+        const SlotState s = st[m.b];
+        const size_t nB = (size_t)d.n * d.B;
+        if (s.phase == PH_ARNOLDI && (pass == 0 || flag[m.b])) {
+            const cd *c = cc + (size_t)m.b * (d.m + 1);
+            for (long long row = (long long)blockIdx.x * m.RW + m.rl; row < d.n; row += (long long)gridDim.x * m.RW) {
+                const size_t e = (size_t)row * d.B + m.b;
+                cd w = W[e];
+                for (int k = 0; k <= s.i; ++k) w = csub(w, cmul(c[k], V[(size_t)k * nB + e]));
+                W[e] = w;
+                nrm += cabs2(w);
+            }
+        }
+    }
+    snrm[threadIdx.x] = nrm;
+    __syncthreads();
+    if (m.ok && m.rl == 0) {
+        double a = 0;
+        for (int r = 0; r < m.RW; ++r) a += snrm[r * m.SB + m.sl];
+        pnrm[(size_t)blockIdx.x * d.B + m.b] = a;
+    }
+}
+
+// copy the coefficients of this pass (H column, or H - previous for pass 2),
+// reduce |w|^2 and decide re-orthogonalisation (DGKS: |w| < |w_before| / sqrt 2)
+__global__ void k_cgs_coef(KrylovDims d, int pass, const SlotState *__restrict__ st, const cd *__restrict__ Hall,
+                           cd *__restrict__ cc, cd *__restrict__ hprev) {
+    const int b = blockIdx.x;
+    if (b >= d.B) return;
+    const SlotState s = st[b];
+    if (s.phase != PH_ARNOLDI) return;
+    const cd *H = Hall + (size_t)b * (d.m + 1) * d.m + (size_t)(d.m + 1) * s.i;
+    cd *c = cc + (size_t)b * (d.m + 1);
+    cd *hp = hprev + (size_t)b * (d.m + 1);
+    for (int k = threadIdx.x; k <= s.i; k += blockDim.x) {
+        if (pass == 0) {
+            c[k] = H[k];
+            hp[k] = H[k];
+        } else {
+            c[k] = csub(H[k], hp[k]);
+        }
+    }
+}
+
+__global__ void k_cgs_norm(KrylovDims d, int pass, const SlotState *__restrict__ st, const double *__restrict__ pnrm,
+                           const double *__restrict__ nav2, int *__restrict__ flag, double *__restrict__ nrm2) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= d.B) return;
+    const SlotState s = st[b];
+    if (s.phase != PH_ARNOLDI) {
+        if (pass == 0) flag[b] = 0;
+        return;
+    }
+    if (pass == 1 && !flag[b]) return;
+    double a = 0;
+    for (int w = 0; w < d.nwg; ++w) a += pnrm[(size_t)w * d.B + b];
+    nrm2[b] = a;
+    if (pass == 0) flag[b] = (a < 0.5 * nav2[b]) ? 1 : 0;  // |w|^2 < |w0|^2 / 2
+}
+
 // per-slot scalar work: Givens rotations, convergence test, least squares
 __global__ void k_slot_update(KrylovDims d, SlotState *__restrict__ st, cd *__restrict__ Hall,
                               cd *__restrict__ csall, cd *__restrict__ snall, cd *__restrict__ gall,
@@ -412,6 +571,25 @@ void launch_store_x(const KrylovDims &d, const int *col, const cd *V, cd *dst, l
     hipLaunchKernelGGL(k_store_x, dim3(g), dim3(256), 0, s, d, col, V, dst, ld);
     HIP_CHECK(hipGetLastError());
 }
+void launch_cgs(const KrylovDims &d, cd *W, const cd *V, const cd *Rhs, const SlotState *st, cd *H, cd *pdot,
+                double *pnrm, double *pnav, double *nrm2, double *nav2, int *flag, cd *cc, cd *hprev, int kmax,
+                hipStream_t s) {
+    const int SB = std::min(d.B, 256);
+    const unsigned ys = (unsigned)((d.B + SB - 1) / SB);
+    const unsigned zs = (unsigned)(kmax / CGS_KC + 1);
+    for (int pass = 0; pass < 2; ++pass) {
+        hipLaunchKernelGGL(k_cgs_dots, dim3((unsigned)d.nwg, ys, zs), dim3(256), 0, s, d, pass, W, V, Rhs, st, flag,
+                           pdot, pnav);
+        hipLaunchKernelGGL(k_cgs_dots_reduce, dim3((unsigned)d.B), dim3(64), 0, s, d, pass, st, flag, H, pdot, pnav,
+                           nav2, nrm2);
+        hipLaunchKernelGGL(k_cgs_coef, dim3((unsigned)d.B), dim3(64), 0, s, d, pass, st, H, cc, hprev);
+        hipLaunchKernelGGL(k_cgs_update, dim3((unsigned)d.nwg, ys), dim3(256), 0, s, d, pass, W, V, st, flag, cc,
+                           pnrm);
+        hipLaunchKernelGGL(k_cgs_norm, dim3((d.B + 63) / 64), dim3(64), 0, s, d, pass, st, pnrm, nav2, flag, nrm2);
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
 void launch_set_inputs(const KrylovDims &d, const SlotState *st, long long *in_off, int *active, hipStream_t s) {
     hipLaunchKernelGGL(k_set_inputs, dim3((d.B + 63) / 64), dim3(64), 0, s, d, st, in_off, active);
     HIP_CHECK(hipGetLastError());

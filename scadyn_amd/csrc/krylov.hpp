@@ -51,6 +51,12 @@ void launch_load_finish(const KrylovDims &d, const int *col, SlotState *st, cons
 // store X of slots with col[b] >= 0 into column col[b] of dst (n x ld)
 void launch_store_x(const KrylovDims &d, const int *col, const cd *V, cd *dst, long long ld, hipStream_t s);
 
+// classical Gram-Schmidt + DGKS for column i of every Arnoldi slot (and the
+// restart residual); kmax = max over slots of i.  pdot holds nwg*(m+1)*B.
+void launch_cgs(const KrylovDims &d, cd *W, const cd *V, const cd *Rhs, const SlotState *st, cd *H, cd *pdot,
+                double *pnrm, double *pnav, double *nrm2, double *nav2, int *flag, cd *cc, cd *hprev, int kmax,
+                hipStream_t s);
+
 // matvec input offsets / activity derived on the device from the slot phases
 void launch_set_inputs(const KrylovDims &d, const SlotState *st, long long *in_off, int *active, hipStream_t s);
 // mark slots with col[b] >= 0 idle

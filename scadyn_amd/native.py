@@ -49,7 +49,7 @@ class COpts(ctypes.Structure):
                 ('M_ex', ctypes.c_int32), ('near_zone', ctypes.c_int32), ('grid_size', ctypes.c_double),
                 ('order', ctypes.c_int32), ('precision', ctypes.c_int32), ('batch', ctypes.c_int32),
                 ('strict', ctypes.c_int32), ('device', ctypes.c_int32), ('fft_pad', ctypes.c_int32),
-                ('refine_tol', ctypes.c_double), ('reserved', ctypes.c_int32 * 8)]
+                ('refine_tol', ctypes.c_double), ('ortho', ctypes.c_int32), ('reserved', ctypes.c_int32 * 7)]
 
 
 _lib = None

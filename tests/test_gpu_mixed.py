@@ -42,14 +42,18 @@ def test_mixed_solve_and_tmatrix(case):
     c, e, param, k = case
     say('fp64 tight')
     op64 = native.Operator(c, e, param, k, tol=1e-11, restart=20)
+    say("mixed, cgs")
+    opc = native.Operator(c, e, param, k, precision=native.PREC_MIXED, tol=1e-5, refine_tol=1e-10, ortho=1)
     say('mixed')
     opm = native.Operator(c, e, param, k, precision=native.PREC_MIXED, tol=1e-5, refine_tol=1e-10)
     rng = np.random.default_rng(6)
     b = rng.standard_normal((3, 3 * e.shape[0])) + 1j * rng.standard_normal((3, 3 * e.shape[0]))
     x64, _ = op64.solve(b)
     xm, _ = opm.solve(b)
+    xc, _ = opc.solve(b)
     for r in range(3):
         assert rel(xm[r], x64[r]) < 1e-8
+        assert rel(xc[r], x64[r]) < 1e-8
     T64, _ = op64.tmatrix(3)
     Tm, _ = opm.tmatrix(3)
     num = np.sqrt(sum(np.linalg.norm(a - b) ** 2 for a, b in zip(Tm, T64)))
@@ -58,3 +62,4 @@ def test_mixed_solve_and_tmatrix(case):
     assert num / den < 1e-6
     op64.close()
     opm.close()
+    opc.close()

@@ -118,3 +118,22 @@ def test_tmatrix(ops):
     for a, b in zip(T, Tref):
         assert rel(a, b) < 1e-7
     assert list(it) == list(itref)
+
+
+def test_solve_cgs_dgks():
+    # classical Gram-Schmidt + DGKS (opts.ortho = 1): same Krylov iterates up to rounding
+    from oracle import pfft_ref
+    from scadyn_amd import native
+    c, e, param, k = sphere_case()
+    say('solve x2 with ortho=1')
+    g = native.Operator(c, e, param, k, fft_pad=1, ortho=1)
+    o = oracle_operator()
+    rng = np.random.default_rng(2)
+    n = 3 * o.g.N
+    b = rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n))
+    x, it = g.solve(b)
+    for r in range(2):
+        xr, ite, _ = pfft_ref.gmres(lambda v: pfft_ref.matvec(o, v), b[r].copy())
+        assert abs(int(it[r]) - ite) <= 1
+        assert rel(x[r], xr) < 1e-7
+    g.close()
