@@ -76,7 +76,10 @@ typedef struct scadyn_opts {
                             reference's, src/gmres_module.f90:241-244); 1 classical
                             Gram-Schmidt with DGKS re-orthogonalisation (two fused
                             passes per iteration)                                       */
-    int32_t reserved[7];
+    int32_t fft_kernel;  /* line-FFT kernels: 0 automatic (register two-stage kernel where
+                            the axis length is R1*R2 with R1, R2 <= 16, else LDS
+                            Stockham), 1 LDS Stockham on every axis                     */
+    int32_t reserved[6];
 } scadyn_opts;
 
 /* Defaults of the reference (src/common.f90:78-89). */

@@ -6,10 +6,11 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libscadyn_amd.so')
-SOURCES = ['api.hip', 'geom.hip', 'setup.hip', 'fft.hip', 'matvec.hip', 'krylov.hip']
+SOURCES = ['api.hip', 'geom.hip', 'setup.hip', 'fft.hip', 'fft_reg.hip', 'matvec.hip', 'krylov.hip']
+OBJDIR = os.path.join(HERE, '_build')
 
 HIPCC_FLAGS = [
-    '-O3', '-std=c++17', '--offload-arch=gfx950', '-fPIC', '-shared',
+    '-O3', '-std=c++17', '--offload-arch=gfx950', '-fPIC',
     # packed FP32 instructions stay off in all device code
     '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
     '-Wno-unused-command-line-argument',
@@ -43,18 +44,56 @@ def build_fortran(verbose=False):
 
 def build(verbose=False):
     hipcc = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    os.makedirs(OBJDIR, exist_ok=True)
+    # one hipcc per translation unit, in parallel, then one link
+    procs, objs = [], []
+    for src in SOURCES:
+        obj = os.path.join(OBJDIR, src.replace('.hip', '.o'))
+        cmd = [hipcc] + HIPCC_FLAGS + ['-c', '-o', obj, os.path.join(CSRC, src)]
+        if verbose:
+            print(' '.join(cmd), flush=True)
+        procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+        objs.append(obj)
+    failed = []
+    for src, p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(out)
+            failed.append(src)
+    if failed:
+        raise RuntimeError('hipcc failed on ' + ', '.join(failed))
     tmp = LIB + '.tmp'
-    cmd = [hipcc] + HIPCC_FLAGS + ['-o', tmp] + srcs
+    cmd = [hipcc, '--offload-arch=gfx950', '-shared', '-fPIC', '-o', tmp] + objs
     if verbose:
         print(' '.join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError('hipcc failed building libscadyn_amd.so')
+        raise RuntimeError('linking libscadyn_amd.so failed')
     os.replace(tmp, LIB)
     build_fortran(verbose)
     return LIB
+
+
+SELFTEST = os.path.join(OBJDIR, 'fft_selftest')
+
+
+def build_selftest(verbose=False):
+    """Host self-test of the register FFT stages (CPU only, AddressSanitizer on
+    the host code); links the library for the pass planner."""
+    hipcc = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
+    os.makedirs(OBJDIR, exist_ok=True)
+    cmd = [hipcc, '-O1', '-g', '-std=c++17', '--offload-arch=gfx950', '-Xarch_host', '-fsanitize=address',
+           '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops', '-Wno-unused-command-line-argument',
+           '-o', SELFTEST, os.path.join(CSRC, 'fft_selftest.hip'), '-L' + HERE, '-lscadyn_amd',
+           '-Wl,-rpath,' + HERE]
+    if verbose:
+        print(' '.join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError('building the FFT self-test failed')
+    return SELFTEST
 
 
 if __name__ == '__main__':

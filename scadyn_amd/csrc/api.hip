@@ -187,11 +187,12 @@ static void fft3d_full(scadyn_operator *op, cd *A, cd *tmp) {
     const int Lx = op->L[0], Ly = op->L[1], Lz = op->L[2];
     const size_t budget = 64 * 1024;
     LinePass p;
-    make_line_pass(p, Lx, Lx, Lx, 1, (long long)Ly * Lz, budget, sizeof(cd));
+    const bool reg = op->opts.fft_kernel == 0;
+    make_line_pass(p, Lx, Lx, Lx, 1, (long long)Ly * Lz, budget, sizeof(cd), reg);
     launch_line_fft<cd>(p, A, tmp, op->d_tw, -1, op->s);
-    make_line_pass(p, Ly, Ly, Ly, Lx, Lz, budget, sizeof(cd));
+    make_line_pass(p, Ly, Ly, Ly, Lx, Lz, budget, sizeof(cd), reg);
     launch_line_fft<cd>(p, tmp, A, op->d_tw + Lx, -1, op->s);
-    make_line_pass(p, Lz, Lz, Lz, (long long)Lx * Ly, 1, budget, sizeof(cd));
+    make_line_pass(p, Lz, Lz, Lz, (long long)Lx * Ly, 1, budget, sizeof(cd), reg);
     launch_line_fft<cd>(p, A, tmp, op->d_tw + Lx + Ly, -1, op->s);
     HIP_CHECK(hipMemcpyAsync(A, tmp, sizeof(cd) * (size_t)Lx * Ly * Lz, hipMemcpyDeviceToDevice, op->s));
 }
@@ -395,13 +396,14 @@ static Workspace &workspace(scadyn_operator *op, int B) {
     w.ret = w.mem.alloc<int>(B);
     w.active = w.mem.alloc<int>(B);
     const size_t budget = 64 * 1024;
+    const bool reg = op->opts.fft_kernel == 0;
     for (int pr = 0; pr < 2; ++pr) {
         const size_t el = pr == 0 ? sizeof(cd) : sizeof(cf);
-        make_line_pass(w.pass[pr][0], Lx, Nx, Lx, nb, (long long)Ny * Nz, budget, el);
-        make_line_pass(w.pass[pr][1], Ly, Ny, Ly, (long long)Lx * nb, Nz, budget, el);
-        make_line_pass(w.pass[pr][2], Lz, Nz, Nz, (long long)Ly * Lx * nb, 1, budget, el);
-        make_line_pass(w.pass[pr][3], Ly, Ly, Ny, (long long)Lx * nb, Nz, budget, el);
-        make_line_pass(w.pass[pr][4], Lx, Lx, Nx, nb, (long long)Ny * Nz, budget, el);
+        make_line_pass(w.pass[pr][0], Lx, Nx, Lx, nb, (long long)Ny * Nz, budget, el, reg);
+        make_line_pass(w.pass[pr][1], Ly, Ny, Ly, (long long)Lx * nb, Nz, budget, el, reg);
+        make_line_pass(w.pass[pr][2], Lz, Nz, Nz, (long long)Ly * Lx * nb, 1, budget, el, reg);
+        make_line_pass(w.pass[pr][3], Ly, Ly, Ny, (long long)Lx * nb, Nz, budget, el, reg);
+        make_line_pass(w.pass[pr][4], Lx, Lx, Nx, nb, (long long)Ny * Nz, budget, el, reg);
     }
     return w;
 }
@@ -1220,6 +1222,7 @@ int scadyn_op_create(const scadyn_mesh *mesh, const scadyn_opts *opts, double k,
         fail(SCADYN_ERR_ARG, "unknown precision mode");
     if (o.maxit < 1 || o.restart < 1) fail(SCADYN_ERR_ARG, "maxit and restart must be >= 1");
     if (o.ortho != 0 && o.ortho != 1) fail(SCADYN_ERR_ARG, "ortho must be 0 (MGS) or 1 (CGS + DGKS)");
+    if (o.fft_kernel != 0 && o.fft_kernel != 1) fail(SCADYN_ERR_ARG, "fft_kernel must be 0 (automatic) or 1 (Stockham)");
     *out = nullptr;
     std::unique_ptr<scadyn_operator> op(new scadyn_operator());
     create_operator(op.get(), *mesh, o, k);

@@ -261,15 +261,40 @@ __global__ __launch_bounds__(256) void k_line(const C *__restrict__ in, C *__res
             out[((size_t)o * p.n_out + j) * p.inner + i] = res[j * p.ti + c];
 }
 
-size_t line_pass_lds(const LinePass &p, size_t elem) { return elem * ((size_t)p.L + 2 * (size_t)p.L * p.ti); }
+size_t line_pass_lds(const LinePass &p, size_t elem) {
+    // twiddles + one buffer (register kernel) or two buffers (Stockham)
+    return elem * ((size_t)p.L + (p.reg ? 1 : 2) * (size_t)p.L * p.ti);
+}
 
 void make_line_pass(LinePass &p, int L, int n_in, int n_out, long long inner, long long outer, size_t lds_budget,
-                    size_t elem) {
+                    size_t elem, bool allow_reg) {
     p.L = L;
     p.n_in = n_in;
     p.n_out = n_out;
     p.inner = inner;
     p.outer = outer;
+    p.reg = 0;
+    p.threads = 256;
+    if (allow_reg) {
+        // most balanced split L = R1 R2 (R1 >= R2) with both radices in the register set
+        int best = 0;
+        for (int r1 = 16; r1 >= 2; --r1)
+            if (L % r1 == 0 && reg_radix_ok(r1) && reg_radix_ok(L / r1) && r1 >= L / r1) best = r1;
+        if (best) {
+            const int r1 = best, r2 = L / best, rmax = std::max(r1, r2);
+            // tile: 16 columns of 16-byte or 32 of 8-byte elements (256 contiguous bytes per row)
+            int ti = elem >= 16 ? 16 : 32;
+            while (ti > 1 && (ti > inner || elem * ((size_t)L + (size_t)L * ti) > lds_budget || ti * rmax > 512))
+                ti /= 2;
+            p.reg = 1;
+            p.nrad = 2;
+            p.rad[0] = r1;
+            p.rad[1] = r2;
+            p.ti = ti;
+            p.threads = ((ti * rmax + 63) / 64) * 64;
+            return;
+        }
+    }
     std::vector<int> r = factor_radices(L);
     if ((int)r.size() > 16) fail(SCADYN_ERR_UNSUPPORTED, "FFT length with too many factors");
     p.nrad = (int)r.size();
@@ -282,6 +307,10 @@ void make_line_pass(LinePass &p, int L, int n_in, int n_out, long long inner, lo
 template <typename C>
 static void launch(const LinePass &p, const C *in, C *out, const C *tw, int sign, const C *fg, const ConvInfo &ci,
                    int mode, hipStream_t s) {
+    if (p.reg) {
+        launch_line_reg<C>(p, in, out, tw, sign, fg, ci, mode, s);
+        return;
+    }
     long long tiles = (p.inner + p.ti - 1) / p.ti;
     long long blocks = tiles * p.outer;
     if (blocks <= 0) return;

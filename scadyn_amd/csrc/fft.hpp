@@ -20,6 +20,8 @@ struct LinePass {
     long long inner = 0;  // contiguous inner extent (elements)
     long long outer = 0;  // number of outer slabs
     int ti = 0;           // tile width (inner columns per workgroup)
+    int reg = 0;          // 1: two-stage register kernel, rad = {R1, R2}
+    int threads = 256;    // workgroup size
 };
 
 // Fg lookup for the fused z-pass: inner index i of the [Ly][Lx][nb] layout
@@ -29,8 +31,14 @@ struct ConvInfo {
     long long nb = 0;  // batch columns per grid point
 };
 
+// allow_reg: use the two-stage register kernel when L = R1 R2 with both
+// radices in the register set
 void make_line_pass(LinePass &p, int L, int n_in, int n_out, long long inner, long long outer, size_t lds_budget,
-                    size_t elem);
+                    size_t elem, bool allow_reg = true);
+bool reg_radix_ok(int r);
+template <typename C>
+void launch_line_reg(const LinePass &p, const C *in, C *out, const C *tw, int sign, const C *fg, const ConvInfo &ci,
+                     int mode, hipStream_t s);
 size_t line_pass_lds(const LinePass &p, size_t elem);
 
 // forward (sign -1) or inverse (sign +1) transform; tw = exp(-2 pi i m / L)

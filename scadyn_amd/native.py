@@ -49,7 +49,8 @@ class COpts(ctypes.Structure):
                 ('M_ex', ctypes.c_int32), ('near_zone', ctypes.c_int32), ('grid_size', ctypes.c_double),
                 ('order', ctypes.c_int32), ('precision', ctypes.c_int32), ('batch', ctypes.c_int32),
                 ('strict', ctypes.c_int32), ('device', ctypes.c_int32), ('fft_pad', ctypes.c_int32),
-                ('refine_tol', ctypes.c_double), ('ortho', ctypes.c_int32), ('reserved', ctypes.c_int32 * 7)]
+                ('refine_tol', ctypes.c_double), ('ortho', ctypes.c_int32), ('fft_kernel', ctypes.c_int32),
+                ('reserved', ctypes.c_int32 * 6)]
 
 
 _lib = None
@@ -116,7 +117,10 @@ def _ip(a):
 def default_opts(**kw):
     o = COpts()
     lib().scadyn_opts_default(ctypes.byref(o))
+    fields = {f[0] for f in COpts._fields_}
     for key, val in kw.items():
+        if key not in fields:
+            raise ScadynError(ERR_ARG, f'unknown option {key!r}')
         setattr(o, key, val)
     return o
 
