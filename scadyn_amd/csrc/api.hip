@@ -382,7 +382,12 @@ static Workspace &workspace(scadyn_operator *op, int B) {
     w.sn = w.mem.alloc<cd>((size_t)B * (m + 1));
     w.g = w.mem.alloc<cd>((size_t)B * (m + 1));
     w.y = w.mem.alloc<cd>((size_t)B * m);
-    w.nwg = 512;
+    {
+        // row-sweep workgroups of the Krylov kernels: up to 2048 (8 waves per
+        // SIMD), each sweeping at least 16 row groups
+        const long long rw = 256 / std::min(B, 256);
+        w.nwg = (int)std::max(1LL, std::min(2048LL, (w.n + rw * 16 - 1) / (rw * 16)));
+    }
     w.nrm2 = w.mem.alloc<double>(B);
     w.nav2 = w.mem.alloc<double>(B);
     w.pnrm = w.mem.alloc<double>((size_t)w.nwg * B);

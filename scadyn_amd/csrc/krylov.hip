@@ -24,6 +24,34 @@ __device__ inline RowSlotMap row_slot_map(int B) {
     return r;
 }
 
+// Deterministic sum over w < nwg of part[w * stride + off] by one workgroup of
+// RED_T threads: fixed per-thread order, then a fixed LDS tree.  Every thread
+// of the workgroup must call it; the result is returned to all of them.
+constexpr int RED_T = 256;
+template <typename T>
+__device__ inline T zero_of();
+template <>
+__device__ inline double zero_of<double>() { return 0.0; }
+template <>
+__device__ inline cd zero_of<cd>() { return cmk(0.0, 0.0); }
+__device__ inline double addv(double a, double b) { return a + b; }
+__device__ inline cd addv(cd a, cd b) { return cadd(a, b); }
+
+template <typename T>
+__device__ T block_sum(const T *__restrict__ part, int nwg, size_t stride, size_t off) {
+    __shared__ T sh[RED_T];
+    T a = zero_of<T>();
+    for (int w = threadIdx.x; w < nwg; w += RED_T) a = addv(a, part[(size_t)w * stride + off]);
+    __syncthreads();  // sh may still be read by a previous call
+    sh[threadIdx.x] = a;
+    __syncthreads();
+    for (int h = RED_T / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) sh[threadIdx.x] = addv(sh[threadIdx.x], sh[threadIdx.x + h]);
+        __syncthreads();
+    }
+    return sh[0];
+}
+
 __global__ __launch_bounds__(256) void k_mgs_pass(KrylovDims d, int k, cd *__restrict__ W,
                                                   const cd *__restrict__ V, const cd *__restrict__ Rhs,
                                                   const SlotState *__restrict__ st, const cd *__restrict__ H,
@@ -89,31 +117,29 @@ __global__ __launch_bounds__(256) void k_mgs_pass(KrylovDims d, int k, cd *__res
     }
 }
 
-__global__ void k_mgs_reduce(KrylovDims d, int k, const SlotState *__restrict__ st, cd *__restrict__ H,
-                             const cd *__restrict__ pdot, const double *__restrict__ pnrm,
-                             const double *__restrict__ pnav, double *__restrict__ nrm2,
-                             double *__restrict__ nav2) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= d.B) return;
+// one workgroup per slot
+__global__ __launch_bounds__(RED_T) void k_mgs_reduce(KrylovDims d, int k, const SlotState *__restrict__ st,
+                                                      cd *__restrict__ H, const cd *__restrict__ pdot,
+                                                      const double *__restrict__ pnrm,
+                                                      const double *__restrict__ pnav, double *__restrict__ nrm2,
+                                                      double *__restrict__ nav2) {
+    const int b = blockIdx.x;
     const SlotState s = st[b];
     const bool arn = s.phase == PH_ARNOLDI && k <= s.i + 1;
     const bool rst = s.phase == PH_RESTART && k == 0;
-    if (!arn && !rst) return;
-    cd a = cmk(0, 0);
-    double x1 = 0, x2 = 0;
-    for (int w = 0; w < d.nwg; ++w) {
-        const size_t o = (size_t)w * d.B + b;
-        a = cadd(a, pdot[o]);
-        x1 += pnrm[o];
-        x2 += pnav[o];
-    }
+    if (!arn && !rst) return;  // uniform over the workgroup
+    const double x1 = block_sum<double>(pnrm, d.nwg, d.B, b);
     if (rst) {
-        nrm2[b] = x1;
+        if (threadIdx.x == 0) nrm2[b] = x1;
         return;
     }
-    if (k <= s.i) H[(size_t)b * (d.m + 1) * d.m + k + (size_t)(d.m + 1) * s.i] = a;
-    if (k == s.i + 1) nrm2[b] = x1;
-    if (k == 0) nav2[b] = x2;
+    const cd a = k <= s.i ? block_sum<cd>(pdot, d.nwg, d.B, b) : cmk(0.0, 0.0);
+    const double x2 = k == 0 ? block_sum<double>(pnav, d.nwg, d.B, b) : 0.0;
+    if (threadIdx.x == 0) {
+        if (k <= s.i) H[(size_t)b * (d.m + 1) * d.m + k + (size_t)(d.m + 1) * s.i] = a;
+        if (k == s.i + 1) nrm2[b] = x1;
+        if (k == 0) nav2[b] = x2;
+    }
 }
 
 // ---- classical Gram-Schmidt with DGKS re-orthogonalisation (opts.ortho = 1) ----
@@ -180,29 +206,30 @@ __global__ __launch_bounds__(256) void k_cgs_dots(KrylovDims d, int pass, cd *__
     }
 }
 
-// sum partial dots into H column i (pass 0 stores, pass 1 adds); |w|^2 of restart slots
-__global__ void k_cgs_dots_reduce(KrylovDims d, int pass, const SlotState *__restrict__ st,
-                                  const int *__restrict__ flag, cd *__restrict__ Hall, const cd *__restrict__ pdot,
-                                  const double *__restrict__ pnav, double *__restrict__ nav2,
-                                  double *__restrict__ nrm2) {
+// sum partial dots into H column i (pass 0 stores, pass 1 adds); |w|^2 of
+// restart slots.  One workgroup per slot.
+__global__ __launch_bounds__(RED_T) void k_cgs_dots_reduce(KrylovDims d, int pass, const SlotState *__restrict__ st,
+                                                           const int *__restrict__ flag, cd *__restrict__ Hall,
+                                                           const cd *__restrict__ pdot,
+                                                           const double *__restrict__ pnav,
+                                                           double *__restrict__ nav2, double *__restrict__ nrm2) {
     const int nk = d.m + 1;
-    const int b = blockIdx.x;  // one block per slot, threads over k
-    if (b >= d.B) return;
+    const int b = blockIdx.x;
     const SlotState s = st[b];
-    if (threadIdx.x == 0 && pass == 0 && (s.phase == PH_ARNOLDI || s.phase == PH_RESTART)) {
-        double a = 0;
-        for (int w = 0; w < d.nwg; ++w) a += pnav[(size_t)w * d.B + b];
-        if (s.phase == PH_ARNOLDI)
-            nav2[b] = a;
-        else
-            nrm2[b] = a;
+    if (pass == 0 && (s.phase == PH_ARNOLDI || s.phase == PH_RESTART)) {
+        const double a = block_sum<double>(pnav, d.nwg, d.B, b);
+        if (threadIdx.x == 0) {
+            if (s.phase == PH_ARNOLDI)
+                nav2[b] = a;
+            else
+                nrm2[b] = a;
+        }
     }
     if (s.phase != PH_ARNOLDI || (pass == 1 && !flag[b])) return;
     cd *H = Hall + (size_t)b * (d.m + 1) * d.m + (size_t)(d.m + 1) * s.i;
-    for (int k = threadIdx.x; k <= s.i; k += blockDim.x) {
-        cd a = cmk(0.0, 0.0);
-        for (int w = 0; w < d.nwg; ++w) a = cadd(a, pdot[((size_t)w * nk + k) * d.B + b]);
-        H[k] = pass == 0 ? a : cadd(H[k], a);
+    for (int k = 0; k <= s.i; ++k) {
+        const cd a = block_sum<cd>(pdot, d.nwg, (size_t)nk * d.B, (size_t)k * d.B + b);
+        if (threadIdx.x == 0) H[k] = pass == 0 ? a : cadd(H[k], a);
     }
 }
 
@@ -259,20 +286,21 @@ __global__ void k_cgs_coef(KrylovDims d, int pass, const SlotState *__restrict__
     }
 }
 
-__global__ void k_cgs_norm(KrylovDims d, int pass, const SlotState *__restrict__ st, const double *__restrict__ pnrm,
-                           const double *__restrict__ nav2, int *__restrict__ flag, double *__restrict__ nrm2) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= d.B) return;
+__global__ __launch_bounds__(RED_T) void k_cgs_norm(KrylovDims d, int pass, const SlotState *__restrict__ st,
+                                                    const double *__restrict__ pnrm, const double *__restrict__ nav2,
+                                                    int *__restrict__ flag, double *__restrict__ nrm2) {
+    const int b = blockIdx.x;
     const SlotState s = st[b];
     if (s.phase != PH_ARNOLDI) {
-        if (pass == 0) flag[b] = 0;
+        if (pass == 0 && threadIdx.x == 0) flag[b] = 0;
         return;
     }
     if (pass == 1 && !flag[b]) return;
-    double a = 0;
-    for (int w = 0; w < d.nwg; ++w) a += pnrm[(size_t)w * d.B + b];
-    nrm2[b] = a;
-    if (pass == 0) flag[b] = (a < 0.5 * nav2[b]) ? 1 : 0;  // |w|^2 < |w0|^2 / 2
+    const double a = block_sum<double>(pnrm, d.nwg, d.B, b);
+    if (threadIdx.x == 0) {
+        nrm2[b] = a;
+        if (pass == 0) flag[b] = (a < 0.5 * nav2[b]) ? 1 : 0;  // |w|^2 < |w0|^2 / 2
+    }
 }
 
 // per-slot scalar work: Givens rotations, convergence test, least squares
@@ -420,12 +448,13 @@ __global__ __launch_bounds__(256) void k_load_rhs(KrylovDims d, const int *__res
     }
 }
 
-__global__ void k_load_finish(KrylovDims d, const int *__restrict__ col, SlotState *__restrict__ st,
-                              const double *__restrict__ pnrm) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= d.B || col[b] < 0) return;
-    double a = 0;
-    for (int w = 0; w < d.nwg; ++w) a += pnrm[(size_t)w * d.B + b];
+// one workgroup per slot
+__global__ __launch_bounds__(RED_T) void k_load_finish(KrylovDims d, const int *__restrict__ col,
+                                                       SlotState *__restrict__ st, const double *__restrict__ pnrm) {
+    const int b = blockIdx.x;
+    if (col[b] < 0) return;
+    const double a = block_sum<double>(pnrm, d.nwg, d.B, b);
+    if (threadIdx.x != 0) return;
     SlotState s;
     s.phase = PH_RESTART;
     s.i = 0;
@@ -538,7 +567,7 @@ void launch_mgs_pass(const KrylovDims &d, int k, cd *W, const cd *V, const cd *R
 }
 void launch_mgs_reduce(const KrylovDims &d, int k, const SlotState *st, cd *H, const cd *pdot,
                        const double *pnrm, const double *pnav, double *nrm2, double *nav2, hipStream_t s) {
-    hipLaunchKernelGGL(k_mgs_reduce, dim3((d.B + 63) / 64), dim3(64), 0, s, d, k, st, H, pdot, pnrm, pnav, nrm2,
+    hipLaunchKernelGGL(k_mgs_reduce, dim3((unsigned)d.B), dim3(RED_T), 0, s, d, k, st, H, pdot, pnrm, pnav, nrm2,
                        nav2);
     HIP_CHECK(hipGetLastError());
 }
@@ -562,7 +591,7 @@ void launch_load_rhs(const KrylovDims &d, const int *col, const cd *src, long lo
 }
 void launch_load_finish(const KrylovDims &d, const int *col, SlotState *st, const double *pnrm,
                         hipStream_t s) {
-    hipLaunchKernelGGL(k_load_finish, dim3((d.B + 63) / 64), dim3(64), 0, s, d, col, st, pnrm);
+    hipLaunchKernelGGL(k_load_finish, dim3((unsigned)d.B), dim3(RED_T), 0, s, d, col, st, pnrm);
     HIP_CHECK(hipGetLastError());
 }
 void launch_store_x(const KrylovDims &d, const int *col, const cd *V, cd *dst, long long ld, hipStream_t s) {
@@ -580,12 +609,12 @@ void launch_cgs(const KrylovDims &d, cd *W, const cd *V, const cd *Rhs, const Sl
     for (int pass = 0; pass < 2; ++pass) {
         hipLaunchKernelGGL(k_cgs_dots, dim3((unsigned)d.nwg, ys, zs), dim3(256), 0, s, d, pass, W, V, Rhs, st, flag,
                            pdot, pnav);
-        hipLaunchKernelGGL(k_cgs_dots_reduce, dim3((unsigned)d.B), dim3(64), 0, s, d, pass, st, flag, H, pdot, pnav,
-                           nav2, nrm2);
+        hipLaunchKernelGGL(k_cgs_dots_reduce, dim3((unsigned)d.B), dim3(RED_T), 0, s, d, pass, st, flag, H, pdot,
+                           pnav, nav2, nrm2);
         hipLaunchKernelGGL(k_cgs_coef, dim3((unsigned)d.B), dim3(64), 0, s, d, pass, st, H, cc, hprev);
         hipLaunchKernelGGL(k_cgs_update, dim3((unsigned)d.nwg, ys), dim3(256), 0, s, d, pass, W, V, st, flag, cc,
                            pnrm);
-        hipLaunchKernelGGL(k_cgs_norm, dim3((d.B + 63) / 64), dim3(64), 0, s, d, pass, st, pnrm, nav2, flag, nrm2);
+        hipLaunchKernelGGL(k_cgs_norm, dim3((unsigned)d.B), dim3(RED_T), 0, s, d, pass, st, pnrm, nav2, flag, nrm2);
     }
     HIP_CHECK(hipGetLastError());
 }

@@ -128,6 +128,130 @@ __global__ __launch_bounds__(256) void k_gather_near(
     }
 }
 
+// ---- wave-uniform variants for B a multiple of 64 ---------------------------
+// One wave64 owns one grid node (scatter) or one tet (gather) for 64 slots:
+// the node/tet index is wave-uniform (readfirstlane), so every per-node or
+// per-tet operand (cube ranges, S*, near-zone entries and blocks) is a scalar
+// load and only the slot-indexed x, grid and y accesses are vector loads,
+// each 64 consecutive complex numbers.
+template <typename C>
+__global__ __launch_bounds__(256) void k_scatter_w64(const cd *__restrict__ xin,
+                                                     const long long *__restrict__ in_off, int B,
+                                                     const int *__restrict__ cube_start,
+                                                     const C *__restrict__ S4, int Nx, int Ny, int Nz,
+                                                     int ncx, int ncy, int ncz, C *__restrict__ grid) {
+    const int lane = threadIdx.x & 63;
+    const int nch = B >> 6;
+    const long long nw = (long long)Nx * Ny * Nz * nch;
+    const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    for (long long w = (long long)blockIdx.x * 4 + wib; w < nw; w += (long long)gridDim.x * 4) {
+        const long long p = w / nch;
+        const int b = (int)(w - p * nch) * 64 + lane;
+        const int px = (int)(p % Nx), py = (int)((p / Nx) % Ny), pz = (int)(p / ((long long)Nx * Ny));
+        const cd *xb = xin + in_off[b] + b;
+        C a0 = Prec<C>::zero(), a1 = a0, a2 = a0, a3 = a0;
+        const int cx0 = max(0, (px - 1) / 2), cx1 = min(ncx - 1, px / 2);
+        const int cy0 = max(0, (py - 1) / 2), cy1 = min(ncy - 1, py / 2);
+        const int cz0 = max(0, (pz - 1) / 2), cz1 = min(ncz - 1, pz / 2);
+        for (int cz = cz0; cz <= cz1; ++cz)
+            for (int cy = cy0; cy <= cy1; ++cy)
+                for (int cx = cx0; cx <= cx1; ++cx) {
+                    const int c = cx + ncx * (cy + ncy * cz);
+                    const int j = (px - 2 * cx) + 3 * (py - 2 * cy) + 9 * (pz - 2 * cz);
+                    const int t1 = cube_start[c + 1];
+                    for (int t = cube_start[c]; t < t1; ++t) {
+                        const C *s = S4 + ((size_t)t * 27 + j) * 4;
+                        const C x0 = Prec<C>::from(xb[(size_t)(3 * t) * B]),
+                                x1 = Prec<C>::from(xb[(size_t)(3 * t + 1) * B]),
+                                x2 = Prec<C>::from(xb[(size_t)(3 * t + 2) * B]);
+                        const C s0 = s[0];
+                        cfma(a0, s0, x0);
+                        cfma(a1, s0, x1);
+                        cfma(a2, s0, x2);
+                        cfma(a3, s[1], x0);
+                        cfma(a3, s[2], x1);
+                        cfma(a3, s[3], x2);
+                    }
+                }
+        C *gp = grid + (size_t)p * 4 * B + b;
+        gp[0] = a0;
+        gp[(size_t)B] = a1;
+        gp[(size_t)2 * B] = a2;
+        gp[(size_t)3 * B] = a3;
+    }
+}
+
+template <typename C>
+__global__ __launch_bounds__(256) void k_gather_near_w64(
+    const C *__restrict__ grid, const cd *__restrict__ xin, const long long *__restrict__ in_off, int B,
+    int N, const int *__restrict__ tet_cube, const C *__restrict__ S4, int Nx, int Ny, int ncx, int ncy,
+    const int64_t *__restrict__ row_start, const int *__restrict__ ent_col,
+    const uint32_t *__restrict__ ent_blk, const cf *__restrict__ blocks, double k2,
+    const int *__restrict__ slot_active, cd *__restrict__ y) {
+    const int lane = threadIdx.x & 63;
+    const int nch = B >> 6;
+    const long long nw = (long long)N * nch;
+    const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    for (long long w = (long long)blockIdx.x * 4 + wib; w < nw; w += (long long)gridDim.x * 4) {
+        const int t = (int)(w / nch);
+        const int b = (int)(w - (long long)t * nch) * 64 + lane;
+        cd *yo = y + (size_t)(3 * t) * B + b;
+        if (slot_active && !slot_active[b]) {
+            yo[0] = yo[(size_t)B] = yo[(size_t)2 * B] = cmk(0, 0);
+            continue;
+        }
+        const int c = tet_cube[t];
+        const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
+        const size_t base = (size_t)(2 * cx) + (size_t)Nx * ((size_t)(2 * cy) + (size_t)Ny * (2 * cz));
+        const C *s = S4 + (size_t)t * 108;
+        C Y0 = Prec<C>::zero(), Y1 = Y0, Y2 = Y0, Z0 = Y0, Z1 = Y0, Z2 = Y0;
+        for (int jz = 0; jz < 3; ++jz)
+            for (int jy = 0; jy < 3; ++jy)
+#pragma unroll
+                for (int jx = 0; jx < 3; ++jx) {
+                    const int j = jx + 3 * jy + 9 * jz;
+                    const size_t node = base + jx + (size_t)Nx * (jy + (size_t)Ny * jz);
+                    const C *g = grid + node * 4 * B + b;
+                    const C g0 = g[0], g1 = g[(size_t)B], g2 = g[(size_t)2 * B], g3 = g[(size_t)3 * B];
+                    const C *sj = s + j * 4;
+                    cfma(Y0, sj[0], g0);
+                    cfma(Y1, sj[0], g1);
+                    cfma(Y2, sj[0], g2);
+                    cfma(Z0, sj[1], g3);
+                    cfma(Z1, sj[2], g3);
+                    cfma(Z2, sj[3], g3);
+                }
+        const cd *xb = xin + in_off[b] + b;
+        C n0 = Prec<C>::zero(), n1 = n0, n2 = n0;
+        const int64_t q1 = row_start[t + 1];
+        for (int64_t q = row_start[t]; q < q1; ++q) {
+            const int col = ent_col[q];
+            const uint32_t eb = ent_blk[q];
+            const cf *a = blocks + (size_t)(eb & 0x7fffffffu) * 9;
+            const C x0 = Prec<C>::from(xb[(size_t)(3 * col) * B]),
+                    x1 = Prec<C>::from(xb[(size_t)(3 * col + 1) * B]),
+                    x2 = Prec<C>::from(xb[(size_t)(3 * col + 2) * B]);
+            C m[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) m[i] = Prec<C>::from(a[i]);
+            if (eb & 0x80000000u) {
+                cfma(n0, m[0], x0); cfma(n0, m[1], x1); cfma(n0, m[2], x2);
+                cfma(n1, m[3], x0); cfma(n1, m[4], x1); cfma(n1, m[5], x2);
+                cfma(n2, m[6], x0); cfma(n2, m[7], x1); cfma(n2, m[8], x2);
+            } else {
+                cfma(n0, m[0], x0); cfma(n0, m[3], x1); cfma(n0, m[6], x2);
+                cfma(n1, m[1], x0); cfma(n1, m[4], x1); cfma(n1, m[7], x2);
+                cfma(n2, m[2], x0); cfma(n2, m[5], x1); cfma(n2, m[8], x2);
+            }
+        }
+        typedef typename Prec<C>::real R;
+        const R kk = (R)k2;
+        yo[0] = Prec<cd>::from(cadd(n0, csub(Z0, cscale(Y0, kk))));
+        yo[(size_t)B] = Prec<cd>::from(cadd(n1, csub(Z1, cscale(Y1, kk))));
+        yo[(size_t)2 * B] = Prec<cd>::from(cadd(n2, csub(Z2, cscale(Y2, kk))));
+    }
+}
+
 static unsigned grid_for(long long tot, int bs) {
     long long g = (tot + bs - 1) / bs;
     return (unsigned)std::min<long long>(std::max<long long>(g, 1), 1LL << 20);
@@ -135,6 +259,13 @@ static unsigned grid_for(long long tot, int bs) {
 
 template <typename C>
 void launch_scatter(const MatvecArgs<C> &a, hipStream_t s) {
+    if (a.B % 64 == 0) {
+        const long long nw = (long long)a.Nx * a.Ny * a.Nz * (a.B / 64);
+        hipLaunchKernelGGL((k_scatter_w64<C>), dim3(grid_for(nw, 4)), dim3(256), 0, s, a.xin, a.in_off, a.B,
+                           a.cube_start, a.S4, a.Nx, a.Ny, a.Nz, a.ncx, a.ncy, a.ncz, a.grid_in);
+        HIP_CHECK(hipGetLastError());
+        return;
+    }
     long long tot = (long long)a.Nx * a.Ny * a.Nz * a.B;
     hipLaunchKernelGGL((k_scatter<C>), dim3(grid_for(tot, 256)), dim3(256), 0, s, a.xin, a.in_off, a.B,
                        a.cube_start, a.S4, a.Nx, a.Ny, a.Nz, a.ncx, a.ncy, a.ncz, a.grid_in);
@@ -143,6 +274,14 @@ void launch_scatter(const MatvecArgs<C> &a, hipStream_t s) {
 
 template <typename C>
 void launch_gather_near(const MatvecArgs<C> &a, hipStream_t s) {
+    if (a.B % 64 == 0) {
+        const long long nw = (long long)a.N * (a.B / 64);
+        hipLaunchKernelGGL((k_gather_near_w64<C>), dim3(grid_for(nw, 4)), dim3(256), 0, s, a.grid_out, a.xin,
+                           a.in_off, a.B, a.N, a.tet_cube, a.S4, a.Nx, a.Ny, a.ncx, a.ncy, a.row_start, a.ent_col,
+                           a.ent_blk, a.blocks, a.k2, a.slot_active, a.y);
+        HIP_CHECK(hipGetLastError());
+        return;
+    }
     long long tot = (long long)a.N * a.B;
     hipLaunchKernelGGL((k_gather_near<C>), dim3(grid_for(tot, 256)), dim3(256), 0, s, a.grid_out, a.xin,
                        a.in_off, a.B, a.N, a.tet_cube, a.S4, a.Nx, a.Ny, a.ncx, a.ncy, a.row_start, a.ent_col,
