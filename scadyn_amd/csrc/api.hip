@@ -154,6 +154,8 @@ struct scadyn_operator {
     cd *d_param = nullptr, *d_S4 = nullptr, *d_fg = nullptr, *d_tw = nullptr;
     cf *d_S4f = nullptr, *d_fgf = nullptr, *d_twf = nullptr;  // complex64 copies (mixed precision)
     int *d_tet_cube = nullptr, *d_cube_start = nullptr, *d_perm = nullptr;
+    int *d_ne_cubes = nullptr;  // non-empty cubes (only with np.cube_rows)
+    int n_ne_cubes = 0;
     cf *d_blocks = nullptr;
     int64_t *d_row_start = nullptr;
     int *d_ent_col = nullptr;
@@ -257,6 +259,13 @@ static void create_operator(scadyn_operator *op, const scadyn_mesh &mesh, const 
     op->d_tet_cube = M.upload(g.tet_cube, s);
     op->d_cube_start = M.upload(g.cube_start, s);
     op->d_perm = M.upload(g.perm, s);
+    if (op->np.cube_rows) {
+        std::vector<int> ne;
+        for (int c = 0; c < g.N_cubes; ++c)
+            if (g.cube_start[c + 1] > g.cube_start[c]) ne.push_back(c);
+        op->n_ne_cubes = (int)ne.size();
+        if (!ne.empty()) op->d_ne_cubes = M.upload(ne, s);
+    }
     {
         std::vector<cd> tw;
         for (int c = 0; c < 3; ++c) {
@@ -468,6 +477,8 @@ static void run_matvec(scadyn_operator *op, Workspace &w, const cd *xin, const i
     a.ent_col = op->d_ent_col;
     a.ent_blk = op->d_ent_blk;
     a.blocks = op->d_blocks;
+    a.ne_cubes = op->d_ne_cubes;
+    a.n_ne = op->n_ne_cubes;
     a.k2 = op->k * op->k;
     a.slot_active = active;
     a.grid_in = g0;

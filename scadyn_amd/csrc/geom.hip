@@ -184,6 +184,49 @@ void build_near_pairs(const HostGeometry &g, NearPairs &np) {
             np.ent_blk[e] = (uint32_t)p | BLK_TRANSPOSE;
         }
     }
+    // order each row by (near-cube offset, column tet) and check that it holds
+    // every tet of the near cubes exactly once
+    np.cube_rows = true;
+    std::vector<int64_t> key;
+    for (int t = 0; t < g.N; ++t) {
+        const int64_t r0 = np.row_start[t], r1 = np.row_start[t + 1];
+        const int ct = g.tet_cube[t];
+        const int tx = ct % g.ncx, ty = (ct / g.ncx) % g.ncy, tz = ct / (g.ncx * g.ncy);
+        key.resize(r1 - r0);
+        for (int64_t e = r0; e < r1; ++e) {
+            const int cc = g.tet_cube[np.ent_col[e]];
+            const int dx = cc % g.ncx - tx, dy = (cc / g.ncx) % g.ncy - ty, dz = cc / (g.ncx * g.ncy) - tz;
+            key[e - r0] = (int64_t)((dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)) * g.N + np.ent_col[e];
+        }
+        // insertion sort: rows are short and close to sorted
+        for (int64_t a = 1; a < r1 - r0; ++a) {
+            const int64_t k = key[a];
+            const int col = np.ent_col[r0 + a];
+            const uint32_t blk = np.ent_blk[r0 + a];
+            int64_t b = a - 1;
+            while (b >= 0 && key[b] > k) {
+                key[b + 1] = key[b];
+                np.ent_col[r0 + b + 1] = np.ent_col[r0 + b];
+                np.ent_blk[r0 + b + 1] = np.ent_blk[r0 + b];
+                --b;
+            }
+            key[b + 1] = k;
+            np.ent_col[r0 + b + 1] = col;
+            np.ent_blk[r0 + b + 1] = blk;
+        }
+        if (!np.cube_rows) continue;
+        int64_t e = r0;
+        for (int dz = -1; dz <= 1 && np.cube_rows; ++dz)
+            for (int dy = -1; dy <= 1 && np.cube_rows; ++dy)
+                for (int dx = -1; dx <= 1 && np.cube_rows; ++dx) {
+                    const int xx = tx + dx, yy = ty + dy, zz = tz + dz;
+                    if (xx < 0 || xx >= g.ncx || yy < 0 || yy >= g.ncy || zz < 0 || zz >= g.ncz) continue;
+                    const int bc = xx + g.ncx * (yy + g.ncy * zz);
+                    for (int c = g.cube_start[bc]; c < g.cube_start[bc + 1]; ++c)
+                        if (e >= r1 || np.ent_col[e++] != c) np.cube_rows = false;
+                }
+        if (e != r1) np.cube_rows = false;
+    }
 }
 
 // Hestenes one-sided Jacobi SVD of A (m x n, column-major), m >= n.

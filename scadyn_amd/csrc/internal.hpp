@@ -118,6 +118,10 @@ struct NearPairs {
     std::vector<int64_t> row_start;
     std::vector<int> ent_col;
     std::vector<uint32_t> ent_blk;
+    // true when every row lists exactly the tets of its cube's near cubes, in
+    // near-cube order (dz, dy, dx ascending) and ascending tet id within a
+    // cube: then the entries of all tets of one cube line up by position
+    bool cube_rows = false;
 };
 constexpr uint32_t BLK_TRANSPOSE = 0x80000000u;
 void build_near_pairs(const HostGeometry &g, NearPairs &np);

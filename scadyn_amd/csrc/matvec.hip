@@ -252,6 +252,203 @@ __global__ __launch_bounds__(256) void k_gather_near_w64(
     }
 }
 
+// ---- on-chip reuse variants (B a multiple of 64) -----------------------------
+// Scatter, one wave per 2x2x2 block of grid nodes (2a..2a+1, 2b..2b+1, 2c..2c+1),
+// ceil(N/2) blocks per axis:
+// every node of the block lies in the boxes of the same <= 8 cubes (a-1..a per
+// axis), so the wave loads each of their tets' x once for all 8 nodes, instead
+// of once per node.  Per node the sum runs over cubes and tets in the same
+// order as k_scatter.
+template <typename C>
+__global__ __launch_bounds__(256) void k_scatter_blk_w64(const cd *__restrict__ xin,
+                                                         const long long *__restrict__ in_off, int B,
+                                                         const int *__restrict__ cube_start,
+                                                         const C *__restrict__ S4, int Nx, int Ny, int Nz,
+                                                         int ncx, int ncy, int ncz, C *__restrict__ grid) {
+    const int lane = threadIdx.x & 63;
+    const int nch = B >> 6;
+    // ceil(N/2) blocks per axis cover every node (nodes beyond the cubes' boxes get zeros)
+    const int nbx = (Nx + 1) >> 1, nby = (Ny + 1) >> 1, nbz = (Nz + 1) >> 1;
+    const long long nw = (long long)nbx * nby * nbz * nch;
+    const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    for (long long w = (long long)blockIdx.x * 4 + wib; w < nw; w += (long long)gridDim.x * 4) {
+        const long long blk = w / nch;
+        const int b = (int)(w - blk * nch) * 64 + lane;
+        const int ba = (int)(blk % nbx), bb = (int)((blk / nbx) % nby), bc = (int)(blk / ((long long)nbx * nby));
+        const cd *xb = xin + in_off[b] + b;
+        C acc[2][2][2][4];
+#pragma unroll
+        for (int iz = 0; iz < 2; ++iz)
+#pragma unroll
+            for (int iy = 0; iy < 2; ++iy)
+#pragma unroll
+                for (int ix = 0; ix < 2; ++ix)
+#pragma unroll
+                    for (int ch = 0; ch < 4; ++ch) acc[iz][iy][ix][ch] = Prec<C>::zero();
+        // cube (ba-1+dx, bb-1+dy, bc-1+dz); node 2ba+ix is in its box when dx = 1
+        // (local index ix) or dx = 0 and ix = 0 (local index 2)
+#pragma unroll
+        for (int dz = 0; dz < 2; ++dz) {
+            const int cz = bc - 1 + dz;
+            if (cz < 0 || cz >= ncz) continue;
+#pragma unroll
+            for (int dy = 0; dy < 2; ++dy) {
+                const int cy = bb - 1 + dy;
+                if (cy < 0 || cy >= ncy) continue;
+#pragma unroll
+                for (int dx = 0; dx < 2; ++dx) {
+                    const int cx = ba - 1 + dx;
+                    if (cx < 0 || cx >= ncx) continue;
+                    const int c = cx + ncx * (cy + ncy * cz);
+                    const int t1 = cube_start[c + 1];
+                    for (int t = cube_start[c]; t < t1; ++t) {
+                        const C x0 = Prec<C>::from(xb[(size_t)(3 * t) * B]),
+                                x1 = Prec<C>::from(xb[(size_t)(3 * t + 1) * B]),
+                                x2 = Prec<C>::from(xb[(size_t)(3 * t + 2) * B]);
+                        const C *st = S4 + (size_t)t * 108;
+#pragma unroll
+                        for (int iz = 0; iz < 2; ++iz) {
+                            if (dz == 0 && iz == 1) continue;
+                            const int jz = dz ? iz : 2;
+#pragma unroll
+                            for (int iy = 0; iy < 2; ++iy) {
+                                if (dy == 0 && iy == 1) continue;
+                                const int jy = dy ? iy : 2;
+#pragma unroll
+                                for (int ix = 0; ix < 2; ++ix) {
+                                    if (dx == 0 && ix == 1) continue;
+                                    const int jx = dx ? ix : 2;
+                                    const C *s = st + (jx + 3 * jy + 9 * jz) * 4;
+                                    C *a = acc[iz][iy][ix];
+                                    const C s0 = s[0];
+                                    cfma(a[0], s0, x0);
+                                    cfma(a[1], s0, x1);
+                                    cfma(a[2], s0, x2);
+                                    cfma(a[3], s[1], x0);
+                                    cfma(a[3], s[2], x1);
+                                    cfma(a[3], s[3], x2);
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int iz = 0; iz < 2; ++iz)
+#pragma unroll
+            for (int iy = 0; iy < 2; ++iy)
+#pragma unroll
+                for (int ix = 0; ix < 2; ++ix) {
+                    const int px = 2 * ba + ix, py = 2 * bb + iy, pz = 2 * bc + iz;
+                    if (px >= Nx || py >= Ny || pz >= Nz) continue;
+                    C *gp = grid + ((size_t)px + (size_t)Nx * ((size_t)py + (size_t)Ny * pz)) * 4 * B + b;
+#pragma unroll
+                    for (int ch = 0; ch < 4; ++ch) gp[(size_t)ch * B] = acc[iz][iy][ix][ch];
+                }
+    }
+}
+
+// Gather + near zone, one wave per non-empty cube: the 27 box nodes are loaded
+// once for all tets of the cube and each near tet's x once for all of them
+// (the rows of a cube's tets list the same columns in the same order).  Up to
+// GT tets per pass; y_c = near_c + Sc'^T g3 - k^2 S^T g_c accumulated directly.
+template <typename C, int GT>
+__global__ __launch_bounds__(256) void k_gather_cube_w64(
+    const C *__restrict__ grid, const cd *__restrict__ xin, const long long *__restrict__ in_off, int B,
+    const int *__restrict__ ne_cubes, int n_ne, const int *__restrict__ cube_start, const C *__restrict__ S4,
+    int Nx, int Ny, int ncx, int ncy, int ncz, const int64_t *__restrict__ row_start,
+    const uint32_t *__restrict__ ent_blk, const cf *__restrict__ blocks, double k2,
+    const int *__restrict__ slot_active, cd *__restrict__ y) {
+    typedef typename Prec<C>::real R;
+    const int lane = threadIdx.x & 63;
+    const int nch = B >> 6;
+    const long long nw = (long long)n_ne * nch;
+    const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const R kk = (R)k2;
+    for (long long w = (long long)blockIdx.x * 4 + wib; w < nw; w += (long long)gridDim.x * 4) {
+        const int ci = (int)(w / nch);
+        const int b = (int)(w - (long long)ci * nch) * 64 + lane;
+        const int c = ne_cubes[ci];
+        const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
+        const int t0 = cube_start[c], t1 = cube_start[c + 1];
+        const bool act = slot_active ? slot_active[b] != 0 : true;
+        const cd *xb = xin + in_off[b] + b;
+        const size_t base = (size_t)(2 * cx) + (size_t)Nx * ((size_t)(2 * cy) + (size_t)Ny * (2 * cz));
+        for (int tp = t0; tp < t1; tp += GT) {
+            const int nt = min(GT, t1 - tp);
+            C acc[GT][3];
+#pragma unroll
+            for (int u = 0; u < GT; ++u) acc[u][0] = acc[u][1] = acc[u][2] = Prec<C>::zero();
+            for (int jz = 0; jz < 3; ++jz)
+                for (int jy = 0; jy < 3; ++jy)
+#pragma unroll
+                    for (int jx = 0; jx < 3; ++jx) {
+                        const int j = jx + 3 * jy + 9 * jz;
+                        const C *g = grid + (base + jx + (size_t)Nx * (jy + (size_t)Ny * jz)) * 4 * B + b;
+                        const C g0 = g[0], g1 = g[(size_t)B], g2 = g[(size_t)2 * B], g3 = g[(size_t)3 * B];
+#pragma unroll
+                        for (int u = 0; u < GT; ++u) {
+                            if (u >= nt) break;
+                            const C *s = S4 + ((size_t)(tp + u) * 27 + j) * 4;
+                            const C s0 = cscale(s[0], -kk);
+                            cfma(acc[u][0], s0, g0);
+                            cfma(acc[u][1], s0, g1);
+                            cfma(acc[u][2], s0, g2);
+                            cfma(acc[u][0], s[1], g3);
+                            cfma(acc[u][1], s[2], g3);
+                            cfma(acc[u][2], s[3], g3);
+                        }
+                    }
+            int64_t q = 0;
+            for (int dz = -1; dz <= 1; ++dz) {
+                const int zz = cz + dz;
+                if (zz < 0 || zz >= ncz) continue;
+                for (int dy = -1; dy <= 1; ++dy) {
+                    const int yy = cy + dy;
+                    if (yy < 0 || yy >= ncy) continue;
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        const int xx = cx + dx;
+                        if (xx < 0 || xx >= ncx) continue;
+                        const int nc = xx + ncx * (yy + ncy * zz);
+                        const int c1 = cube_start[nc + 1];
+                        for (int col = cube_start[nc]; col < c1; ++col, ++q) {
+                            const C x0 = Prec<C>::from(xb[(size_t)(3 * col) * B]),
+                                    x1 = Prec<C>::from(xb[(size_t)(3 * col + 1) * B]),
+                                    x2 = Prec<C>::from(xb[(size_t)(3 * col + 2) * B]);
+#pragma unroll
+                            for (int u = 0; u < GT; ++u) {
+                                if (u >= nt) break;
+                                const uint32_t eb = ent_blk[row_start[tp + u] + q];
+                                const cf *a = blocks + (size_t)(eb & 0x7fffffffu) * 9;
+                                C m[9];
+#pragma unroll
+                                for (int i = 0; i < 9; ++i) m[i] = Prec<C>::from(a[i]);
+                                if (eb & 0x80000000u) {
+                                    cfma(acc[u][0], m[0], x0); cfma(acc[u][0], m[1], x1); cfma(acc[u][0], m[2], x2);
+                                    cfma(acc[u][1], m[3], x0); cfma(acc[u][1], m[4], x1); cfma(acc[u][1], m[5], x2);
+                                    cfma(acc[u][2], m[6], x0); cfma(acc[u][2], m[7], x1); cfma(acc[u][2], m[8], x2);
+                                } else {
+                                    cfma(acc[u][0], m[0], x0); cfma(acc[u][0], m[3], x1); cfma(acc[u][0], m[6], x2);
+                                    cfma(acc[u][1], m[1], x0); cfma(acc[u][1], m[4], x1); cfma(acc[u][1], m[7], x2);
+                                    cfma(acc[u][2], m[2], x0); cfma(acc[u][2], m[5], x1); cfma(acc[u][2], m[8], x2);
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < GT; ++u) {
+                if (u >= nt) break;
+                cd *yo = y + (size_t)(3 * (tp + u)) * B + b;
+#pragma unroll
+                for (int r = 0; r < 3; ++r) yo[(size_t)r * B] = act ? Prec<cd>::from(acc[u][r]) : cmk(0, 0);
+            }
+        }
+    }
+}
+
 static unsigned grid_for(long long tot, int bs) {
     long long g = (tot + bs - 1) / bs;
     return (unsigned)std::min<long long>(std::max<long long>(g, 1), 1LL << 20);
@@ -260,8 +457,8 @@ static unsigned grid_for(long long tot, int bs) {
 template <typename C>
 void launch_scatter(const MatvecArgs<C> &a, hipStream_t s) {
     if (a.B % 64 == 0) {
-        const long long nw = (long long)a.Nx * a.Ny * a.Nz * (a.B / 64);
-        hipLaunchKernelGGL((k_scatter_w64<C>), dim3(grid_for(nw, 4)), dim3(256), 0, s, a.xin, a.in_off, a.B,
+        const long long nw = (long long)((a.Nx + 1) / 2) * ((a.Ny + 1) / 2) * ((a.Nz + 1) / 2) * (a.B / 64);
+        hipLaunchKernelGGL((k_scatter_blk_w64<C>), dim3(grid_for(nw, 4)), dim3(256), 0, s, a.xin, a.in_off, a.B,
                            a.cube_start, a.S4, a.Nx, a.Ny, a.Nz, a.ncx, a.ncy, a.ncz, a.grid_in);
         HIP_CHECK(hipGetLastError());
         return;
@@ -274,6 +471,14 @@ void launch_scatter(const MatvecArgs<C> &a, hipStream_t s) {
 
 template <typename C>
 void launch_gather_near(const MatvecArgs<C> &a, hipStream_t s) {
+    if (a.B % 64 == 0 && a.ne_cubes) {
+        const long long nw = (long long)a.n_ne * (a.B / 64);
+        hipLaunchKernelGGL((k_gather_cube_w64<C, 8>), dim3(grid_for(nw, 4)), dim3(256), 0, s, a.grid_out, a.xin,
+                           a.in_off, a.B, a.ne_cubes, a.n_ne, a.cube_start, a.S4, a.Nx, a.Ny, a.ncx, a.ncy, a.ncz,
+                           a.row_start, a.ent_blk, a.blocks, a.k2, a.slot_active, a.y);
+        HIP_CHECK(hipGetLastError());
+        return;
+    }
     if (a.B % 64 == 0) {
         const long long nw = (long long)a.N * (a.B / 64);
         hipLaunchKernelGGL((k_gather_near_w64<C>), dim3(grid_for(nw, 4)), dim3(256), 0, s, a.grid_out, a.xin,

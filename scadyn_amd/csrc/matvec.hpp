@@ -20,6 +20,10 @@ struct MatvecArgs {
     const int *ent_col = nullptr;
     const uint32_t *ent_blk = nullptr;
     const cf *blocks = nullptr;
+    // non-empty cubes; set only when the near-zone rows are in cube order
+    // (NearPairs::cube_rows), which the per-cube gather relies on
+    const int *ne_cubes = nullptr;
+    int n_ne = 0;
     double k2 = 0;
     const int *slot_active = nullptr;  // optional per-slot mask
     C *grid_in = nullptr;              // scatter output [Nz][Ny][Nx][4][B]
