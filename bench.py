@@ -105,6 +105,47 @@ def cpu_baseline(op_gpu, c, e, param, k, matvecs_per_col, budget_s=25.0):
                        f'GPU; operator arrays exported from the device build ({prep:.0f} s prep)')}
 
 
+def compare(args, c, e, param, k, nmax, desc):
+    """Mixed precision vs FP64 on the same workload at a matched accuracy.
+
+    FP64: the reference's GMRES to --tol64.  Mixed: FP32-matvec GMRES to
+    --inner-tol inside FP64 iterative refinement to --refine-tol.  Reports the
+    columns/s of both (operator setup excluded), their ratio and the relative
+    Frobenius difference of the two T-matrices."""
+    from scadyn_amd import native
+    ncol = 2 * ((nmax + 1) ** 2 - 1)
+    common = dict(batch=args.batch, ortho=args.ortho, maxit=args.maxit, restart=args.restart)
+    res = {}
+    for name, prec, tol in (('fp64', native.PREC_FP64, args.tol64), ('mixed', native.PREC_MIXED, args.inner_tol)):
+        op = native.Operator(c, e, param, k, precision=prec, tol=tol, refine_tol=args.refine_tol, **common)
+        t = time.time()
+        T, it = op.tmatrix(nmax)
+        el = time.time() - t
+        inf = op.info()
+        res[name] = {'T': T, 'seconds': el, 'columns_per_s': ncol / el, 'mean_iters': float(it.mean()),
+                     'max_iters': int(it.max()), 'rounds': inf['outer_rounds']}
+        print(f"[bench] {name}: {el:.2f} s, {ncol / el:.3f} columns/s, mean iterations {it.mean():.1f}, "
+              f"rounds {inf['outer_rounds']}", file=sys.stderr, flush=True)
+        op.close()
+    num = np.sqrt(sum(np.linalg.norm(a - b) ** 2 for a, b in zip(res['mixed']['T'], res['fp64']['T'])))
+    den = np.sqrt(sum(np.linalg.norm(b) ** 2 for b in res['fp64']['T']))
+    err = float(num / den)
+    for r in res.values():
+        r.pop('T')
+    out = {'metric': METRIC, 'value': res['mixed']['columns_per_s'], 'unit': 'columns/s', 'n_gpus': 1,
+           'steps': 1, 'warmup': 0, 'ms_per_step': 1000.0 * res['mixed']['seconds'], 'higher_is_better': True,
+           'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32 matvec + f64 refinement',
+           'data': 'synthetic (seeded mesh generator)',
+           'config': {'workload': desc, 'n_tet': int(e.shape[0]), 'columns': ncol, 'nmax': nmax,
+                      'fp64': res['fp64'], 'mixed': res['mixed'],
+                      'speedup_mixed_over_fp64': res['mixed']['columns_per_s'] / res['fp64']['columns_per_s'],
+                      'T_rel_frobenius_mixed_vs_fp64': err,
+                      'gmres': {'tol64': args.tol64, 'inner_tol': args.inner_tol, 'refine_tol': args.refine_tol,
+                                'maxit': args.maxit, 'restart': args.restart,
+                                'ortho': 'cgs+dgks' if args.ortho else 'mgs'}}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -118,6 +159,14 @@ def main():
     ap.add_argument('--refine-tol', type=float, default=1e-9)
     ap.add_argument('--max-active', type=int, default=8)
     ap.add_argument('--ortho', type=int, default=0, help='0 MGS (reference), 1 CGS + DGKS')
+    ap.add_argument('--maxit', type=int, default=50, help='Krylov dimension per cycle (reference default 50)')
+    ap.add_argument('--restart', type=int, default=4, help='restart cycles (reference default 4)')
+    ap.add_argument('--compare', action='store_true',
+                    help='mixed precision vs FP64 at a matched accuracy: both T-matrices, their '
+                         'relative difference and the columns/s of each')
+    ap.add_argument('--tol64', type=float, default=1e-8, help='--compare: FP64 GMRES tolerance')
+    ap.add_argument('--inner-tol', type=float, default=1e-4, help='--compare: inner FP32 GMRES tolerance')
+    ap.add_argument('--nmax', type=int, default=0, help='override the workload expansion order')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU path is not built.',
@@ -134,11 +183,17 @@ def main():
         print(json.dumps(out), flush=True)
         return
     c, e, param, k, nmax, desc = make_workload(args.workload)
+    if args.nmax:
+        nmax = args.nmax
+        desc += f' (nmax overridden to {nmax})'
     print(f'[bench] workload {args.workload}: {e.shape[0]} tets, nmax {nmax}', file=sys.stderr, flush=True)
+    if args.compare:
+        compare(args, c, e, param, k, nmax, desc)
+        return
     t0 = time.time()
     prec = native.PREC_MIXED if args.precision == 'mixed' else native.PREC_FP64
     op = native.Operator(c, e, param, k, batch=args.batch, precision=prec, tol=args.tol,
-                         refine_tol=args.refine_tol, ortho=args.ortho)
+                         refine_tol=args.refine_tol, ortho=args.ortho, maxit=args.maxit, restart=args.restart)
     setup_s = time.time() - t0
     info = op.info()
     print(f'[bench] setup {setup_s:.2f} s: {info}', file=sys.stderr, flush=True)
@@ -189,6 +244,9 @@ def main():
                    'grid': [info['Nx'], info['Ny'], info['Nz']], 'fft': [info['Lx'], info['Ly'], info['Lz']],
                    'near_blocks': info['n_blocks'], 'batch': info['batch'], 'setup_s': setup_s,
                    'mean_gmres_iters': float(it.mean()), 'matvecs_per_column': mv_per_col,
+                   'gmres': {'tol': args.tol, 'maxit': args.maxit, 'restart': args.restart,
+                             'ortho': 'cgs+dgks' if args.ortho else 'mgs',
+                             'columns_at_iteration_cap': int(np.sum(it >= args.maxit * args.restart))},
                    'columns_per_s_with_setup': ncol * args.steps / (elapsed + setup_s * args.steps)},
         'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': achieved / HBM_PEAK_GBS, 'traffic': None,

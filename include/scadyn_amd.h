@@ -114,7 +114,9 @@ int scadyn_op_create(const scadyn_mesh *mesh, const scadyn_opts *opts, double k,
 void scadyn_op_destroy(scadyn_operator *op);
 
 /* Sizes: info[0..15] = N_tet, Nx, Ny, Nz, N_cubes, N_tet_cube, n_blocks,
- * Lx, Ly, Lz, batch, device bytes, 0... ; delta and box_delta in dinfo[0..1]. */
+ * Lx, Ly, Lz, batch, device bytes, refinement rounds of the last mixed-precision
+ * solve, 1 if the near-zone rows are in cube order (per-cube gather), 0... ;
+ * delta, box_delta and k in dinfo[0..2]. */
 int scadyn_op_info(const scadyn_operator *op, int64_t info[16], double dinfo[4]);
 
 #define SCADYN_EXPORT_S       1  /* S,Sx,Sy,Sz: complex [4][N][27], reference tet order  */

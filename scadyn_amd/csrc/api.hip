@@ -1269,6 +1269,8 @@ int scadyn_op_info(const scadyn_operator *op, int64_t info[16], double dinfo[4])
         info[9] = op->L[2];
         info[10] = op->ws ? op->ws->B : 0;
         info[11] = (int64_t)(op->mem.bytes + (op->ws ? op->ws->mem.bytes : 0));
+        info[12] = op->last_outer_rounds;
+        info[13] = op->np.cube_rows ? 1 : 0;
     }
     if (dinfo) {
         dinfo[0] = op->g.delta;

@@ -128,7 +128,7 @@ __global__ __launch_bounds__(256) void k_gather_near(
     }
 }
 
-// ---- wave-uniform variants for B a multiple of 64 ---------------------------
+// ---- wave-uniform variants (slots in chunks of 64) ---------------------------
 // One wave64 owns one grid node (scatter) or one tet (gather) for 64 slots:
 // the node/tet index is wave-uniform (readfirstlane), so every per-node or
 // per-tet operand (cube ranges, S*, near-zone entries and blocks) is a scalar
@@ -141,12 +141,14 @@ __global__ __launch_bounds__(256) void k_scatter_w64(const cd *__restrict__ xin,
                                                      const C *__restrict__ S4, int Nx, int Ny, int Nz,
                                                      int ncx, int ncy, int ncz, C *__restrict__ grid) {
     const int lane = threadIdx.x & 63;
-    const int nch = B >> 6;
+    const int nch = (B + 63) >> 6;  // last chunk may be partial: lanes >= B load slot B-1, store nothing
     const long long nw = (long long)Nx * Ny * Nz * nch;
     const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     for (long long w = (long long)blockIdx.x * 4 + wib; w < nw; w += (long long)gridDim.x * 4) {
         const long long p = w / nch;
-        const int b = (int)(w - p * nch) * 64 + lane;
+        const int braw = (int)(w - p * nch) * 64 + lane;
+        const bool valid = braw < B;
+        const int b = valid ? braw : B - 1;
         const int px = (int)(p % Nx), py = (int)((p / Nx) % Ny), pz = (int)(p / ((long long)Nx * Ny));
         const cd *xb = xin + in_off[b] + b;
         C a0 = Prec<C>::zero(), a1 = a0, a2 = a0, a3 = a0;
@@ -173,11 +175,13 @@ __global__ __launch_bounds__(256) void k_scatter_w64(const cd *__restrict__ xin,
                         cfma(a3, s[3], x2);
                     }
                 }
-        C *gp = grid + (size_t)p * 4 * B + b;
-        gp[0] = a0;
-        gp[(size_t)B] = a1;
-        gp[(size_t)2 * B] = a2;
-        gp[(size_t)3 * B] = a3;
+        if (valid) {
+            C *gp = grid + (size_t)p * 4 * B + b;
+            gp[0] = a0;
+            gp[(size_t)B] = a1;
+            gp[(size_t)2 * B] = a2;
+            gp[(size_t)3 * B] = a3;
+        }
     }
 }
 
@@ -189,12 +193,15 @@ __global__ __launch_bounds__(256) void k_gather_near_w64(
     const uint32_t *__restrict__ ent_blk, const cf *__restrict__ blocks, double k2,
     const int *__restrict__ slot_active, cd *__restrict__ y) {
     const int lane = threadIdx.x & 63;
-    const int nch = B >> 6;
+    const int nch = (B + 63) >> 6;  // last chunk may be partial: lanes >= B load slot B-1, store nothing
     const long long nw = (long long)N * nch;
     const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     for (long long w = (long long)blockIdx.x * 4 + wib; w < nw; w += (long long)gridDim.x * 4) {
         const int t = (int)(w / nch);
-        const int b = (int)(w - (long long)t * nch) * 64 + lane;
+        const int braw = (int)(w - (long long)t * nch) * 64 + lane;
+        const bool valid = braw < B;
+        const int b = valid ? braw : B - 1;
+        if (!valid) continue;
         cd *yo = y + (size_t)(3 * t) * B + b;
         if (slot_active && !slot_active[b]) {
             yo[0] = yo[(size_t)B] = yo[(size_t)2 * B] = cmk(0, 0);
@@ -252,7 +259,7 @@ __global__ __launch_bounds__(256) void k_gather_near_w64(
     }
 }
 
-// ---- on-chip reuse variants (B a multiple of 64) -----------------------------
+// ---- on-chip reuse variants (slots in chunks of 64) --------------------------
 // Scatter, one wave per 2x2x2 block of grid nodes (2a..2a+1, 2b..2b+1, 2c..2c+1),
 // ceil(N/2) blocks per axis:
 // every node of the block lies in the boxes of the same <= 8 cubes (a-1..a per
@@ -266,14 +273,16 @@ __global__ __launch_bounds__(256) void k_scatter_blk_w64(const cd *__restrict__ 
                                                          const C *__restrict__ S4, int Nx, int Ny, int Nz,
                                                          int ncx, int ncy, int ncz, C *__restrict__ grid) {
     const int lane = threadIdx.x & 63;
-    const int nch = B >> 6;
+    const int nch = (B + 63) >> 6;  // last chunk may be partial: lanes >= B load slot B-1, store nothing
     // ceil(N/2) blocks per axis cover every node (nodes beyond the cubes' boxes get zeros)
     const int nbx = (Nx + 1) >> 1, nby = (Ny + 1) >> 1, nbz = (Nz + 1) >> 1;
     const long long nw = (long long)nbx * nby * nbz * nch;
     const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     for (long long w = (long long)blockIdx.x * 4 + wib; w < nw; w += (long long)gridDim.x * 4) {
         const long long blk = w / nch;
-        const int b = (int)(w - blk * nch) * 64 + lane;
+        const int braw = (int)(w - blk * nch) * 64 + lane;
+        const bool valid = braw < B;
+        const int b = valid ? braw : B - 1;
         const int ba = (int)(blk % nbx), bb = (int)((blk / nbx) % nby), bc = (int)(blk / ((long long)nbx * nby));
         const cd *xb = xin + in_off[b] + b;
         C acc[2][2][2][4];
@@ -341,7 +350,7 @@ __global__ __launch_bounds__(256) void k_scatter_blk_w64(const cd *__restrict__ 
 #pragma unroll
                 for (int ix = 0; ix < 2; ++ix) {
                     const int px = 2 * ba + ix, py = 2 * bb + iy, pz = 2 * bc + iz;
-                    if (px >= Nx || py >= Ny || pz >= Nz) continue;
+                    if (!valid || px >= Nx || py >= Ny || pz >= Nz) continue;
                     C *gp = grid + ((size_t)px + (size_t)Nx * ((size_t)py + (size_t)Ny * pz)) * 4 * B + b;
 #pragma unroll
                     for (int ch = 0; ch < 4; ++ch) gp[(size_t)ch * B] = acc[iz][iy][ix][ch];
@@ -362,13 +371,15 @@ __global__ __launch_bounds__(256) void k_gather_cube_w64(
     const int *__restrict__ slot_active, cd *__restrict__ y) {
     typedef typename Prec<C>::real R;
     const int lane = threadIdx.x & 63;
-    const int nch = B >> 6;
+    const int nch = (B + 63) >> 6;  // last chunk may be partial: lanes >= B load slot B-1, store nothing
     const long long nw = (long long)n_ne * nch;
     const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const R kk = (R)k2;
     for (long long w = (long long)blockIdx.x * 4 + wib; w < nw; w += (long long)gridDim.x * 4) {
         const int ci = (int)(w / nch);
-        const int b = (int)(w - (long long)ci * nch) * 64 + lane;
+        const int braw = (int)(w - (long long)ci * nch) * 64 + lane;
+        const bool valid = braw < B;
+        const int b = valid ? braw : B - 1;
         const int c = ne_cubes[ci];
         const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
         const int t0 = cube_start[c], t1 = cube_start[c + 1];
@@ -440,7 +451,7 @@ __global__ __launch_bounds__(256) void k_gather_cube_w64(
             }
 #pragma unroll
             for (int u = 0; u < GT; ++u) {
-                if (u >= nt) break;
+                if (u >= nt || !valid) break;
                 cd *yo = y + (size_t)(3 * (tp + u)) * B + b;
 #pragma unroll
                 for (int r = 0; r < 3; ++r) yo[(size_t)r * B] = act ? Prec<cd>::from(acc[u][r]) : cmk(0, 0);
@@ -449,6 +460,9 @@ __global__ __launch_bounds__(256) void k_gather_cube_w64(
     }
 }
 
+// slot-chunk (wave64-per-item) kernels when at most a quarter of the lanes idle
+static bool wave_slots(int B) { return B % 64 == 0 || B % 64 >= 48; }
+
 static unsigned grid_for(long long tot, int bs) {
     long long g = (tot + bs - 1) / bs;
     return (unsigned)std::min<long long>(std::max<long long>(g, 1), 1LL << 20);
@@ -456,8 +470,8 @@ static unsigned grid_for(long long tot, int bs) {
 
 template <typename C>
 void launch_scatter(const MatvecArgs<C> &a, hipStream_t s) {
-    if (a.B % 64 == 0) {
-        const long long nw = (long long)((a.Nx + 1) / 2) * ((a.Ny + 1) / 2) * ((a.Nz + 1) / 2) * (a.B / 64);
+    if (wave_slots(a.B)) {
+        const long long nw = (long long)((a.Nx + 1) / 2) * ((a.Ny + 1) / 2) * ((a.Nz + 1) / 2) * ((a.B + 63) / 64);
         hipLaunchKernelGGL((k_scatter_blk_w64<C>), dim3(grid_for(nw, 4)), dim3(256), 0, s, a.xin, a.in_off, a.B,
                            a.cube_start, a.S4, a.Nx, a.Ny, a.Nz, a.ncx, a.ncy, a.ncz, a.grid_in);
         HIP_CHECK(hipGetLastError());
@@ -471,16 +485,16 @@ void launch_scatter(const MatvecArgs<C> &a, hipStream_t s) {
 
 template <typename C>
 void launch_gather_near(const MatvecArgs<C> &a, hipStream_t s) {
-    if (a.B % 64 == 0 && a.ne_cubes) {
-        const long long nw = (long long)a.n_ne * (a.B / 64);
+    if (wave_slots(a.B) && a.ne_cubes) {
+        const long long nw = (long long)a.n_ne * ((a.B + 63) / 64);
         hipLaunchKernelGGL((k_gather_cube_w64<C, 8>), dim3(grid_for(nw, 4)), dim3(256), 0, s, a.grid_out, a.xin,
                            a.in_off, a.B, a.ne_cubes, a.n_ne, a.cube_start, a.S4, a.Nx, a.Ny, a.ncx, a.ncy, a.ncz,
                            a.row_start, a.ent_blk, a.blocks, a.k2, a.slot_active, a.y);
         HIP_CHECK(hipGetLastError());
         return;
     }
-    if (a.B % 64 == 0) {
-        const long long nw = (long long)a.N * (a.B / 64);
+    if (wave_slots(a.B)) {
+        const long long nw = (long long)a.N * ((a.B + 63) / 64);
         hipLaunchKernelGGL((k_gather_near_w64<C>), dim3(grid_for(nw, 4)), dim3(256), 0, s, a.grid_out, a.xin,
                            a.in_off, a.B, a.N, a.tet_cube, a.S4, a.Nx, a.Ny, a.ncx, a.ncy, a.row_start, a.ent_col,
                            a.ent_blk, a.blocks, a.k2, a.slot_active, a.y);
