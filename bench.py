@@ -28,6 +28,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = 'T-matrix columns/s, mixed vs FP64 pFFT-GMRES at 1e-6 rel err; p99 latency'
 HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFLOPS = 78.6
 
 from scadyn_amd.workloads import make_workload, request_stream  # noqa: E402
 
@@ -231,6 +232,14 @@ def main():
                       / max(1e-12, tim[n]['ms'] / max(tim[n]['launches'], 1) * 1e-3) / 1e9}
                   for n in mv_classes}
     per_kernel['krylov'] = {'ms_total': tim['krylov']['ms']}
+    tg = tim['tgemm']
+    gemm = None
+    if tg['launches']:
+        tflops = tg['bytes'] / (tg['ms'] * 1e-3) / 1e12
+        gemm = {'kernel': 'tgemm (FP64 MFMA 16x16x4)', 'avg_us': 1000.0 * tg['ms'] / tg['launches'],
+                'achieved': tflops, 'peak': FP64_MFMA_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                'frac': tflops / FP64_MFMA_PEAK_TFLOPS}
+        per_kernel['tgemm'] = gemm
     print('[bench] kernels ' + json.dumps(per_kernel), file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
@@ -252,6 +261,7 @@ def main():
                      'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
                      'avg_launch_us': avg_ms * 1000.0, 'bytes_per_launch': bytes_per_launch},
         'cpu_baseline': cpu,
+        'assembly_gemm': gemm,
     }
     print(json.dumps(out), flush=True)
     op.close()

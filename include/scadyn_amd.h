@@ -153,7 +153,9 @@ int scadyn_op_tmatrix(scadyn_operator *op, int32_t nmax,
 
 /* Kernel timing: when enabled, every launch of the hot kernels is bracketed
  * by HIP events on the operator's stream.  timings[i] = {total ms, launches,
- * algorithmic bytes} for kernel class i (see SCADYN_KCLASS_*). */
+ * algorithmic bytes} for kernel class i (see SCADYN_KCLASS_*); for the T-matrix
+ * projection GEMM (SCADYN_KCLASS_TGEMM) the third number is floating-point
+ * operations instead of bytes.  FP64 and FP32 matvec launches share a class. */
 #define SCADYN_KCLASS_SCATTER  0
 #define SCADYN_KCLASS_FFTX     1
 #define SCADYN_KCLASS_FFTY     2
@@ -162,7 +164,8 @@ int scadyn_op_tmatrix(scadyn_operator *op, int32_t nmax,
 #define SCADYN_KCLASS_IFFTX    5
 #define SCADYN_KCLASS_GATHER   6  /* gather + near-zone + combine */
 #define SCADYN_KCLASS_KRYLOV   7
-#define SCADYN_KCLASS_COUNT    8
+#define SCADYN_KCLASS_TGEMM    8  /* T = mat^H X on the FP64 matrix cores */
+#define SCADYN_KCLASS_COUNT    9
 int scadyn_op_profile(scadyn_operator *op, int32_t enable);
 int scadyn_op_timings(scadyn_operator *op, double *timings /* [COUNT][3] */);
 

@@ -487,8 +487,6 @@ static void run_matvec(scadyn_operator *op, Workspace &w, const cd *xin, const i
     const double c = sizeof(C);
     const double nB = 4.0 * w.B;
     const double Lx = op->L[0], Ly = op->L[1], Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
-    const int ko = pr ? SCADYN_KCLASS_COUNT : 0;  // fp32 launches are accounted separately
-    (void)ko;
     {
         ProfScope ps(op, SCADYN_KCLASS_SCATTER, c * (Nx * Ny * Nz * nB + 27.0 * 4 * g.N) + 16.0 * 3 * g.N * w.B);
         launch_scatter<C>(a, op->s);
@@ -975,7 +973,10 @@ struct TmatJob {
         DevMem t2;
         cd *wk = t2.alloc<cd>(work);
         cd *T = t2.alloc<cd>((size_t)ncol * ncol);
-        launch_tgemm(mat, X, n, ncol, T, wk, work, s);
+        {
+            ProfScope ps(op, SCADYN_KCLASS_TGEMM, 8.0 * (double)ncol * ncol * (double)n);
+            launch_tgemm(mat, X, n, ncol, T, wk, work, s);
+        }
         std::vector<cd> hT((size_t)ncol * ncol);
         HIP_CHECK(hipMemcpyAsync(hT.data(), T, sizeof(cd) * hT.size(), hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));

@@ -522,27 +522,58 @@ __global__ void k_retire(KrylovDims d, const int *__restrict__ col, SlotState *_
 }
 
 // ---- T = mat^H X, split over row chunks -----------------------------------------
+// T = mat^H X on the FP64 matrix cores (v_mfma_f64_16x16x4f64).  Complex
+// products as four real ones: Re T = Mr^T Xr + Mi^T Xi, Im T = Mr^T Xi - Mi^T Xr.
+// A workgroup (4 waves) owns a 64 x 64 tile of T and one chunk of rows; each
+// wave a 32 x 32 quarter as 2 x 2 blocks of 16 x 16.  Operand layout of the
+// 16x16x4 instruction: A[i][k] and B[k][j] in lane i + 16 k (resp. j + 16 k);
+// D[i][j] in lane j + 16 (i / 4), element i % 4.  blockIdx.z (the row chunk)
+// is the slowest grid index, so the 36-ish tiles of one chunk run together
+// and read its rows of mat and X from HBM about once.
+typedef double v4d __attribute__((ext_vector_type(4)));
+
 __global__ __launch_bounds__(256) void k_tgemm(const cd *__restrict__ A, const cd *__restrict__ X, long long n,
                                                int ncol, long long chunk, cd *__restrict__ part) {
-    __shared__ cd sa[32][17], sx[32][17];
-    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-    const int i = blockIdx.x * 16 + ty, j = blockIdx.y * 16 + tx;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int kk = lane >> 4, ii = lane & 15;
+    const int i0 = blockIdx.x * 64 + (wv >> 1) * 32, j0 = blockIdx.y * 64 + (wv & 1) * 32;
     const long long r0 = blockIdx.z * chunk, r1 = std::min(n, r0 + chunk);
-    cd acc = cmk(0, 0);
-    for (long long rb = r0; rb < r1; rb += 32) {
-        for (int e = threadIdx.x; e < 32 * 16; e += 256) {
-            const int rr = e / 16, cc = e % 16;
-            const long long row = rb + rr;
-            const int ia = blockIdx.x * 16 + cc, jx = blockIdx.y * 16 + cc;
-            sa[rr][cc] = (row < r1 && ia < ncol) ? A[(size_t)row * ncol + ia] : cmk(0, 0);
-            sx[rr][cc] = (row < r1 && jx < ncol) ? X[(size_t)row * ncol + jx] : cmk(0, 0);
-        }
-        __syncthreads();
-#pragma unroll 8
-        for (int rr = 0; rr < 32; ++rr) acc = cadd(acc, cmulc(sa[rr][ty], sx[rr][tx]));
-        __syncthreads();
+    v4d accR[2][2], accI[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) accR[a][b] = accI[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+    const int ca0 = i0 + ii, ca1 = i0 + 16 + ii, cb0 = j0 + ii, cb1 = j0 + 16 + ii;
+#pragma unroll 2
+    for (long long r = r0; r < r1; r += 4) {
+        const long long row = r + kk;
+        const bool rok = row < r1;
+        const cd *ar = A + (size_t)row * ncol, *xr = X + (size_t)row * ncol;
+        const cd z = cmk(0.0, 0.0);
+        const cd a0 = (rok && ca0 < ncol) ? ar[ca0] : z, a1 = (rok && ca1 < ncol) ? ar[ca1] : z;
+        const cd b0 = (rok && cb0 < ncol) ? xr[cb0] : z, b1 = (rok && cb1 < ncol) ? xr[cb1] : z;
+        const double am[2][2] = {{a0.x, a0.y}, {a1.x, a1.y}};
+        const double bm[2][2] = {{b0.x, b0.y}, {b1.x, b1.y}};
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                accR[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(am[a][0], bm[b][0], accR[a][b], 0, 0, 0);
+                accR[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(am[a][1], bm[b][1], accR[a][b], 0, 0, 0);
+                accI[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(am[a][0], bm[b][1], accI[a][b], 0, 0, 0);
+                accI[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-am[a][1], bm[b][0], accI[a][b], 0, 0, 0);
+            }
     }
-    if (i < ncol && j < ncol) part[((size_t)blockIdx.z * ncol + i) * ncol + j] = acc;
+    cd *out = part + (size_t)blockIdx.z * ncol * ncol;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int i = i0 + 16 * a + 4 * kk + e, j = j0 + 16 * b + ii;
+                if (i < ncol && j < ncol) out[(size_t)i * ncol + j] = cmk(accR[a][b][e], accI[a][b][e]);
+            }
 }
 
 __global__ void k_tgemm_reduce(const cd *__restrict__ part, int nz, int ncol, cd *__restrict__ T) {
@@ -629,11 +660,14 @@ void launch_retire(const KrylovDims &d, const int *col, SlotState *st, hipStream
 }
 void launch_tgemm(const cd *mat, const cd *X, long long n, int ncol, cd *T, cd *work, size_t work_elems,
                   hipStream_t s) {
-    const int tiles = (ncol + 15) / 16;
-    long long nz = std::max<long long>(1, std::min<long long>((long long)(work_elems / ((size_t)ncol * ncol)),
-                                                               std::max<long long>(1, 4096 / (tiles * tiles))));
-    nz = std::min<long long>(nz, std::max<long long>(1, n / 64));
-    const long long chunk = (n + nz - 1) / nz;
+    // 64 x 64 tiles; row chunks (multiples of 4 rows) so that about 2048
+    // workgroups run, limited by the partial-sum workspace
+    const int tiles = (ncol + 63) / 64;
+    long long nz = std::max<long long>(1, 2048 / (tiles * tiles));
+    nz = std::min<long long>(nz, std::max<long long>(1, (long long)(work_elems / ((size_t)ncol * ncol))));
+    nz = std::min<long long>(nz, std::max<long long>(1, n / 256));
+    long long chunk = (n + nz - 1) / nz;
+    chunk = (chunk + 3) / 4 * 4;
     nz = (n + chunk - 1) / chunk;
     hipLaunchKernelGGL(k_tgemm, dim3(tiles, tiles, (unsigned)nz), dim3(256), 0, s, mat, X, n, ncol, chunk, work);
     HIP_CHECK(hipGetLastError());

@@ -15,7 +15,7 @@ OK = 0
 ERR_ARG, ERR_ORIENTATION, ERR_GRID, ERR_SINGULAR, ERR_NOCONV, ERR_MEMORY, ERR_DEVICE, ERR_UNSUPPORTED = range(1, 9)
 PREC_FP64, PREC_MIXED = 0, 1
 EXPORT_S, EXPORT_BLOCKS, EXPORT_BLKIND, EXPORT_FG = 1, 2, 3, 4
-KCLASS = ['scatter', 'fftx', 'ffty', 'fftz_conv', 'iffty', 'ifftx', 'gather_near', 'krylov']
+KCLASS = ['scatter', 'fftx', 'ffty', 'fftz_conv', 'iffty', 'ifftx', 'gather_near', 'krylov', 'tgemm']
 
 # every symbol include/scadyn_amd.h declares
 EXPORTED = ['scadyn_opts_default', 'scadyn_last_error', 'scadyn_tmat_solve', 'scadyn_op_create',
