@@ -146,6 +146,12 @@ int scadyn_op_solve(scadyn_operator *op, int32_t nrhs, const double *rhs, double
  * (vswf2constant, src/T_matrix.f90:280-305). */
 int scadyn_op_vswf(scadyn_operator *op, int32_t nmax, double *mat);
 
+/* Projection of given solutions onto the regular VSWFs, T_mat = mat^H X
+ * (src/T_matrix.f90:266, before the i k^3 scale of :270-275): X is (3N, 2nm)
+ * column-major in reference tet order, T (2nm, 2nm) column-major.  Runs the
+ * FP64 matrix-core GEMM of scadyn_op_tmatrix. */
+int scadyn_op_project(scadyn_operator *op, int32_t nmax, const double *X, double *T);
+
 /* T-matrix blocks of this operator (compute_T_matrix, src/T_matrix.f90:240-277). */
 int scadyn_op_tmatrix(scadyn_operator *op, int32_t nmax,
                       double *Taa, double *Tab, double *Tba, double *Tbb,

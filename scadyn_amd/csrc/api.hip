@@ -566,10 +566,19 @@ static int auto_batch(scadyn_operator *op, int ncols) {
     HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
     size_t per = slot_bytes(op, op->opts.maxit);
     size_t cap = (size_t)(0.6 * (double)free_b) / std::max<size_t>(per, 1);
-    int B = (int)std::min<size_t>(cap, 64);
-    B = std::max(1, std::min(B, ncols));
     if (cap == 0) fail(SCADYN_ERR_MEMORY, "not enough device memory for one RHS slot");
-    return B;
+    // Up to 64 slots; above 64 columns, split them into equal generations of at
+    // most 128 slots when each generation fills its last 64-slot chunk to at
+    // least 48 (the slot-chunk kernels then idle at most a quarter of a chunk)
+    // and memory allows: no nearly empty final generation of refills.
+    int B = std::min(64, ncols);
+    if (ncols > 64) {
+        const int gen = (ncols + 127) / 128;
+        const int Bg = (ncols + gen - 1) / gen;
+        if ((Bg % 64 == 0 || Bg % 64 >= 48) && (size_t)Bg <= cap) B = Bg;
+    }
+    B = (int)std::min<size_t>((size_t)B, cap);
+    return std::max(1, B);
 }
 
 struct SolveSpec {
@@ -1390,6 +1399,41 @@ int scadyn_op_vswf(scadyn_operator *op, int32_t nmax, double *mat) {
         for (int c = 0; c < 3; ++c)
             for (int j = 0; j < ncol; ++j)
                 o[(size_t)j * n + 3 * (size_t)op->g.perm[i] + c] = h[(size_t)(3 * i + c) * ncol + j];
+    ABI_CATCH
+}
+
+int scadyn_op_project(scadyn_operator *op, int32_t nmax, const double *X, double *T) {
+    ABI_TRY
+    if (!op || !X || !T || nmax < 1) fail(SCADYN_ERR_ARG, "bad argument");
+    HIP_CHECK(hipSetDevice(op->device));
+    const int N = op->g.N;
+    const int nm = (nmax + 1) * (nmax + 1) - 1, ncol = 2 * nm;
+    const size_t n = 3 * (size_t)N;
+    // X column-major (3N, 2nm) in reference tet order -> internal rows, row-major n x ncol
+    const cd *xi = (const cd *)X;
+    std::vector<cd> hx(n * ncol);
+    for (int i = 0; i < N; ++i)
+        for (int c = 0; c < 3; ++c)
+            for (int j = 0; j < ncol; ++j)
+                hx[(size_t)(3 * i + c) * ncol + j] = xi[(size_t)j * n + 3 * (size_t)op->g.perm[i] + c];
+    DevMem tmp;
+    cd *mat = tmp.alloc<cd>(n * ncol);
+    launch_vswf(op->d_tcoord, op->d_vol, N, nmax, op->k, mat, ncol, op->s);
+    cd *dx = tmp.alloc<cd>(n * ncol);
+    h2d_sync(dx, hx.data(), sizeof(cd) * hx.size(), op->s);
+    const size_t work = std::max<size_t>((size_t)ncol * ncol * 64, 1 << 20);
+    cd *wk = tmp.alloc<cd>(work);
+    cd *dT = tmp.alloc<cd>((size_t)ncol * ncol);
+    {
+        ProfScope ps(op, SCADYN_KCLASS_TGEMM, 8.0 * (double)ncol * ncol * (double)n);
+        launch_tgemm(mat, dx, (long long)n, ncol, dT, wk, work, op->s);
+    }
+    std::vector<cd> hT((size_t)ncol * ncol);
+    HIP_CHECK(hipMemcpyAsync(hT.data(), dT, sizeof(cd) * hT.size(), hipMemcpyDeviceToHost, op->s));
+    HIP_CHECK(hipStreamSynchronize(op->s));
+    cd *o = (cd *)T;
+    for (int i = 0; i < ncol; ++i)
+        for (int j = 0; j < ncol; ++j) o[(size_t)j * ncol + i] = hT[(size_t)i * ncol + j];
     ABI_CATCH
 }
 

@@ -526,8 +526,8 @@ __global__ void k_retire(KrylovDims d, const int *__restrict__ col, SlotState *_
 // products as four real ones: Re T = Mr^T Xr + Mi^T Xi, Im T = Mr^T Xi - Mi^T Xr.
 // A workgroup (4 waves) owns a 64 x 64 tile of T and one chunk of rows; each
 // wave a 32 x 32 quarter as 2 x 2 blocks of 16 x 16.  Operand layout of the
-// 16x16x4 instruction: A[i][k] and B[k][j] in lane i + 16 k (resp. j + 16 k);
-// D[i][j] in lane j + 16 (i / 4), element i % 4.  blockIdx.z (the row chunk)
+// 16x16x4 f64 instruction: A[i][k] and B[k][j] in lane i + 16 k (resp. j + 16 k);
+// D[i][j] in lane j + 16 (i % 4), element i / 4.  blockIdx.z (the row chunk)
 // is the slowest grid index, so the 36-ish tiles of one chunk run together
 // and read its rows of mat and X from HBM about once.
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -571,7 +571,7 @@ __global__ __launch_bounds__(256) void k_tgemm(const cd *__restrict__ A, const c
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int i = i0 + 16 * a + 4 * kk + e, j = j0 + 16 * b + ii;
+                const int i = i0 + 16 * a + kk + 4 * e, j = j0 + 16 * b + ii;
                 if (i < ncol && j < ncol) out[(size_t)i * ncol + j] = cmk(accR[a][b][e], accI[a][b][e]);
             }
 }

@@ -20,7 +20,7 @@ KCLASS = ['scatter', 'fftx', 'ffty', 'fftz_conv', 'iffty', 'ifftx', 'gather_near
 # every symbol include/scadyn_amd.h declares
 EXPORTED = ['scadyn_opts_default', 'scadyn_last_error', 'scadyn_tmat_solve', 'scadyn_op_create',
             'scadyn_op_destroy', 'scadyn_op_info', 'scadyn_op_export', 'scadyn_op_matvec', 'scadyn_op_matvec_prec',
-            'scadyn_op_solve', 'scadyn_op_vswf', 'scadyn_op_tmatrix', 'scadyn_op_profile',
+            'scadyn_op_solve', 'scadyn_op_vswf', 'scadyn_op_project', 'scadyn_op_tmatrix', 'scadyn_op_profile',
             'scadyn_op_timings', 'scadyn_footprint', 'scadyn_server_create', 'scadyn_server_submit',
             'scadyn_server_status', 'scadyn_server_wait', 'scadyn_server_destroy']
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
@@ -82,6 +82,7 @@ def lib():
     L.scadyn_op_matvec_prec.argtypes = [P, ctypes.c_int32, ctypes.c_int32, dp, dp]
     L.scadyn_op_solve.argtypes = [P, ctypes.c_int32, dp, dp, ip]
     L.scadyn_op_vswf.argtypes = [P, ctypes.c_int32, dp]
+    L.scadyn_op_project.argtypes = [P, ctypes.c_int32, dp, dp]
     L.scadyn_op_tmatrix.argtypes = [P, ctypes.c_int32, dp, dp, dp, dp, ip]
     L.scadyn_op_profile.argtypes = [P, ctypes.c_int32]
     L.scadyn_op_timings.argtypes = [P, dp]
@@ -215,6 +216,16 @@ class Operator:
         mat = np.zeros((2 * nm, 3 * self.N), np.complex128)  # column-major (3N, 2nm)
         check(lib().scadyn_op_vswf(self.h, nmax, _dp(mat.view(np.float64))))
         return mat.T
+
+    def project(self, nmax, X):
+        """mat^H X for X of shape (2nm, 3N) (one solution per row)."""
+        nm = (nmax + 1) ** 2 - 1
+        Xc = np.ascontiguousarray(np.asarray(X, dtype=np.complex128))
+        if Xc.shape != (2 * nm, 3 * self.N):
+            raise ScadynError(ERR_ARG, f'X must have shape {(2 * nm, 3 * self.N)}')
+        T = np.zeros((2 * nm, 2 * nm), np.complex128)  # filled column-major
+        check(lib().scadyn_op_project(self.h, nmax, _dp(Xc.view(np.float64)), _dp(T.view(np.float64))))
+        return T.T.copy()
 
     def tmatrix(self, nmax):
         nm = (nmax + 1) ** 2 - 1

@@ -137,3 +137,22 @@ def test_solve_cgs_dgks():
         assert abs(int(it[r]) - ite) <= 1
         assert rel(x[r], xr) < 1e-7
     g.close()
+
+
+@pytest.mark.parametrize('nmax', [1, 3, 8])
+def test_project_mfma_gemm(nmax):
+    """mat^H X on the FP64 matrix cores vs numpy, at 2nm = 6, 30 and 160 columns
+    (one partial 16x16 block up to several 64x64 tiles with a ragged edge).
+    Tolerance 1e-13 relative: FP64 products, different summation order."""
+    from scadyn_amd import native
+    c, e, param, k = sphere_case()
+    g = native.Operator(c, e, param, k)
+    nm = (nmax + 1) ** 2 - 1
+    rng = np.random.default_rng(20 + nmax)
+    X = rng.standard_normal((2 * nm, 3 * e.shape[0])) + 1j * rng.standard_normal((2 * nm, 3 * e.shape[0]))
+    say(f'project nmax {nmax} ({2 * nm} columns)')
+    T = g.project(nmax, X)
+    mat = g.vswf(nmax)                      # (3N, 2nm)
+    ref = mat.conj().T @ X.T                # (2nm, 2nm)
+    assert rel(T, ref) < 1e-13
+    g.close()
