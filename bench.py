@@ -29,6 +29,7 @@ sys.path.insert(0, ROOT)
 METRIC = 'T-matrix columns/s, mixed vs FP64 pFFT-GMRES at 1e-6 rel err; p99 latency'
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFLOPS = 78.6
+ORTHO_NAMES = {0: 'mgs', 1: 'cgs+dgks', 2: 'block-mgs'}
 
 from scadyn_amd.workloads import make_workload, request_stream  # noqa: E402
 
@@ -143,7 +144,7 @@ def compare(args, c, e, param, k, nmax, desc):
                       'T_rel_frobenius_mixed_vs_fp64': err,
                       'gmres': {'tol64': args.tol64, 'inner_tol': args.inner_tol, 'refine_tol': args.refine_tol,
                                 'maxit': args.maxit, 'restart': args.restart,
-                                'ortho': 'cgs+dgks' if args.ortho else 'mgs'}}}
+                                'ortho': ORTHO_NAMES[args.ortho]}}}
     print(json.dumps(out), flush=True)
 
 
@@ -159,7 +160,7 @@ def main():
     ap.add_argument('--tol', type=float, default=1e-5)
     ap.add_argument('--refine-tol', type=float, default=1e-9)
     ap.add_argument('--max-active', type=int, default=8)
-    ap.add_argument('--ortho', type=int, default=0, help='0 MGS (reference), 1 CGS + DGKS')
+    ap.add_argument('--ortho', type=int, default=0, help='0 MGS (reference), 1 CGS + DGKS, 2 block MGS')
     ap.add_argument('--maxit', type=int, default=50, help='Krylov dimension per cycle (reference default 50)')
     ap.add_argument('--restart', type=int, default=4, help='restart cycles (reference default 4)')
     ap.add_argument('--compare', action='store_true',
@@ -254,7 +255,7 @@ def main():
                    'near_blocks': info['n_blocks'], 'batch': info['batch'], 'setup_s': setup_s,
                    'mean_gmres_iters': float(it.mean()), 'matvecs_per_column': mv_per_col,
                    'gmres': {'tol': args.tol, 'maxit': args.maxit, 'restart': args.restart,
-                             'ortho': 'cgs+dgks' if args.ortho else 'mgs',
+                             'ortho': ORTHO_NAMES[args.ortho],
                              'columns_at_iteration_cap': int(np.sum(it >= args.maxit * args.restart))},
                    'columns_per_s_with_setup': ncol * args.steps / (elapsed + setup_s * args.steps)},
         'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',

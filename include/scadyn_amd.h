@@ -73,9 +73,12 @@ typedef struct scadyn_opts {
                             1: the reference's 2N                                       */
     double refine_tol;   /* mixed precision: outer FP64 relative residual target        */
     int32_t ortho;       /* Arnoldi orthogonalisation: 0 modified Gram-Schmidt (the
-                            reference's, src/gmres_module.f90:241-244); 1 classical
-                            Gram-Schmidt with DGKS re-orthogonalisation (two fused
-                            passes per iteration)                                       */
+                            reference's, src/gmres_module.f90:241-244), one pass per
+                            basis vector; 1 classical Gram-Schmidt with DGKS
+                            re-orthogonalisation (two fused passes per iteration);
+                            2 block MGS: the MGS coefficients 8 basis vectors per pass
+                            with an in-pass Gram correction (same recurrence, about
+                            half the HBM traffic of 0)                                  */
     int32_t fft_kernel;  /* line-FFT kernels: 0 automatic (register two-stage kernel where
                             the axis length is R1*R2 with R1, R2 <= 16, else LDS
                             Stockham), 1 LDS Stockham on every axis                     */

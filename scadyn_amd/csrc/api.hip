@@ -401,9 +401,9 @@ static Workspace &workspace(scadyn_operator *op, int B) {
     w.nav2 = w.mem.alloc<double>(B);
     w.pnrm = w.mem.alloc<double>((size_t)w.nwg * B);
     w.pnav = w.mem.alloc<double>((size_t)w.nwg * B);
-    w.pdot = w.mem.alloc<cd>((size_t)w.nwg * (m + 1) * B);
+    w.pdot = w.mem.alloc<cd>((size_t)w.nwg * std::max(m + 1, bmgs_partials()) * B);
     w.flag = w.mem.alloc<int>(B);
-    w.cc = w.mem.alloc<cd>((size_t)B * (m + 1));
+    w.cc = w.mem.alloc<cd>((size_t)B * std::max(m + 1, bmgs_partials() + 2));
     w.hprev = w.mem.alloc<cd>((size_t)B * (m + 1));
     w.in_off = w.mem.alloc<long long>(B);
     w.col = w.mem.alloc<int>(B);
@@ -586,7 +586,7 @@ struct SolveSpec {
     bool x0zero = false;          // x0 = 0 (correction equations) instead of x0 = rhs
     double tol = 1e-5;
     int restart = 4;
-    int ortho = 0;                // 0 MGS (reference), 1 CGS + DGKS
+    int ortho = 0;                // 0 MGS (reference), 1 CGS + DGKS, 2 block MGS
 };
 
 // Pinned host buffer (page-locked, so async copies never read a reused buffer).
@@ -707,6 +707,9 @@ struct ColumnSolver {
             if (spec.ortho == 1) {
                 launch_cgs(d, w->W, w->V, w->Rhs, w->st, w->H, w->pdot, w->pnrm, w->pnav, w->nrm2, w->nav2, w->flag,
                            w->cc, w->hprev, std::max(K - 1, 0), s);
+            } else if (spec.ortho == 2) {
+                launch_bmgs(d, w->W, w->V, w->Rhs, w->st, w->H, w->pdot, w->pnrm, w->pnav, w->cc, w->nrm2, w->nav2,
+                            std::max(K - 1, 0), s);
             } else {
                 for (int kk = 0; kk <= K; ++kk) {
                     launch_mgs_pass(d, kk, w->W, w->V, w->Rhs, w->st, w->H, w->pdot, w->pnrm, w->pnav, s);
@@ -1247,7 +1250,8 @@ int scadyn_op_create(const scadyn_mesh *mesh, const scadyn_opts *opts, double k,
     if (o.precision != SCADYN_PREC_FP64 && o.precision != SCADYN_PREC_MIXED)
         fail(SCADYN_ERR_ARG, "unknown precision mode");
     if (o.maxit < 1 || o.restart < 1) fail(SCADYN_ERR_ARG, "maxit and restart must be >= 1");
-    if (o.ortho != 0 && o.ortho != 1) fail(SCADYN_ERR_ARG, "ortho must be 0 (MGS) or 1 (CGS + DGKS)");
+    if (o.ortho < 0 || o.ortho > 2)
+        fail(SCADYN_ERR_ARG, "ortho must be 0 (MGS), 1 (CGS + DGKS) or 2 (block MGS)");
     if (o.fft_kernel != 0 && o.fft_kernel != 1) fail(SCADYN_ERR_ARG, "fft_kernel must be 0 (automatic) or 1 (Stockham)");
     *out = nullptr;
     std::unique_ptr<scadyn_operator> op(new scadyn_operator());

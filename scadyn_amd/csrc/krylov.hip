@@ -142,6 +142,153 @@ __global__ __launch_bounds__(RED_T) void k_mgs_reduce(KrylovDims d, int k, const
     }
 }
 
+// ---- block modified Gram-Schmidt (opts.ortho = 2) ------------------------------
+// The MGS recurrence h_k = <v_k, w - sum_{l<k} h_l v_l> evaluated BS basis vectors
+// per pass: pass p applies the deferred update w -= sum h_l v_l of block p-1 and
+// takes, from the same w, the dots d_j = <v_j, w> of block p together with the
+// block's Gram entries G_jl = <v_j, v_l> (l < j); then
+//   h_j = d_j - sum_{l<j in block} G_jl h_l
+// is exactly the MGS coefficient (forward substitution, k_bmgs_coef).  A pass
+// reads w once and 2 BS basis vectors, against 2 per coefficient for one MGS
+// pass per vector.  Pass nb = ceil((i+1)/BS) applies the last update and takes
+// |w|^2; pass 0 also takes |w|^2 before orthogonalisation (the reference's
+// re-orthogonalisation test) and turns restarting slots' W into rhs - A x.
+constexpr int BS = 8;
+constexpr int BQ = BS + BS * (BS - 1) / 2;  // dots + strictly lower Gram entries
+__device__ inline int gram_idx(int j, int l) { return BS + j * (j - 1) / 2 + l; }  // l < j
+
+__global__ __launch_bounds__(256) void k_bmgs_pass(KrylovDims d, int p, cd *__restrict__ W,
+                                                   const cd *__restrict__ V, const cd *__restrict__ Rhs,
+                                                   const SlotState *__restrict__ st, const cd *__restrict__ H,
+                                                   cd *__restrict__ pq, double *__restrict__ pnrm,
+                                                   double *__restrict__ pnav) {
+    __shared__ cd sq[256];
+    __shared__ double sn1[256], sn2[256];
+    RowSlotMap m = row_slot_map(d.B);
+    cd acc[BQ];
+#pragma unroll
+    for (int q = 0; q < BQ; ++q) acc[q] = cmk(0.0, 0.0);
+    double nrm = 0.0, nav = 0.0;
+    if (m.ok) {
+        const SlotState s = st[m.b];
+        const size_t nB = (size_t)d.n * d.B;
+        const int nb = (s.i + BS) / BS;  // blocks of basis vectors 0..i
+        const bool rst = s.phase == PH_RESTART && p == 0;
+        const bool arn = s.phase == PH_ARNOLDI && p <= nb;
+        if (rst || arn) {
+            const int ulo = (p - 1) * BS, uhi = p >= 1 ? min(p * BS, s.i + 1) : 0;
+            const int dlo = p * BS, dhi = p < nb ? min((p + 1) * BS, s.i + 1) : 0;
+            cd h[BS];
+            const cd *Hc = H + (size_t)m.b * (d.m + 1) * d.m + (size_t)(d.m + 1) * s.i;
+#pragma unroll
+            for (int u = 0; u < BS; ++u) h[u] = (arn && ulo + u < uhi) ? Hc[ulo + u] : cmk(0.0, 0.0);
+            for (long long row = (long long)blockIdx.x * m.RW + m.rl; row < d.n; row += (long long)gridDim.x * m.RW) {
+                const size_t e = (size_t)row * d.B + m.b;
+                cd w = W[e];
+                if (rst) {
+                    w = csub(Rhs[e], w);
+                    W[e] = w;
+                    nrm += cabs2(w);
+                    continue;
+                }
+                if (p == 0) nav += cabs2(w);
+                if (p >= 1) {
+#pragma unroll
+                    for (int u = 0; u < BS; ++u)
+                        if (ulo + u < uhi) w = csub(w, cmul(h[u], V[(size_t)(ulo + u) * nB + e]));
+                    W[e] = w;
+                }
+                if (dhi > dlo) {
+                    cd v[BS];
+#pragma unroll
+                    for (int u = 0; u < BS; ++u) v[u] = (dlo + u < dhi) ? V[(size_t)(dlo + u) * nB + e] : cmk(0.0, 0.0);
+#pragma unroll
+                    for (int j = 0; j < BS; ++j) {
+                        acc[j] = cadd(acc[j], cmulc(v[j], w));
+#pragma unroll
+                        for (int l = 0; l < j; ++l) acc[gram_idx(j, l)] = cadd(acc[gram_idx(j, l)], cmulc(v[j], v[l]));
+                    }
+                } else {
+                    nrm += cabs2(w);
+                }
+            }
+        }
+    }
+    // workgroup partials: [blockIdx.x][q][B] (q < BQ), norms [blockIdx.x][B]
+    sn1[threadIdx.x] = nrm;
+    sn2[threadIdx.x] = nav;
+    __syncthreads();
+    if (m.ok && m.rl == 0) {
+        double a = 0, b = 0;
+        for (int r = 0; r < m.RW; ++r) {
+            a += sn1[r * m.SB + m.sl];
+            b += sn2[r * m.SB + m.sl];
+        }
+        pnrm[(size_t)blockIdx.x * d.B + m.b] = a;
+        pnav[(size_t)blockIdx.x * d.B + m.b] = b;
+    }
+    for (int q = 0; q < BQ; ++q) {
+        __syncthreads();
+        sq[threadIdx.x] = acc[q];
+        __syncthreads();
+        if (m.ok && m.rl == 0) {
+            cd a = cmk(0.0, 0.0);
+            for (int r = 0; r < m.RW; ++r) a = cadd(a, sq[r * m.SB + m.sl]);
+            pq[((size_t)blockIdx.x * BQ + q) * d.B + m.b] = a;
+        }
+    }
+}
+
+// one workgroup per (partial q, slot): q < BQ dots/Gram, BQ |w|^2, BQ+1 |w0|^2
+__global__ __launch_bounds__(RED_T) void k_bmgs_reduce(KrylovDims d, int p, const SlotState *__restrict__ st,
+                                                       const cd *__restrict__ pq, const double *__restrict__ pnrm,
+                                                       const double *__restrict__ pnav, cd *__restrict__ red) {
+    const int q = blockIdx.x, b = blockIdx.y;
+    const SlotState s = st[b];
+    const int nb = (s.i + BS) / BS;
+    const bool rst = s.phase == PH_RESTART && p == 0;
+    const bool arn = s.phase == PH_ARNOLDI && p <= nb;
+    if (!rst && !arn) return;
+    cd a = cmk(0.0, 0.0);
+    if (q < BQ) {
+        if (arn && p < nb) a = block_sum<cd>(pq, d.nwg, (size_t)BQ * d.B, (size_t)q * d.B + b);
+    } else if (q == BQ) {
+        a.x = block_sum<double>(pnrm, d.nwg, d.B, b);
+    } else {
+        if (arn && p == 0) a.x = block_sum<double>(pnav, d.nwg, d.B, b);
+    }
+    if (threadIdx.x == 0) red[(size_t)b * (BQ + 2) + q] = a;
+}
+
+// forward substitution h_j = d_j - sum_{l<j} G_jl h_l into H column i; norms
+__global__ void k_bmgs_coef(KrylovDims d, int p, const SlotState *__restrict__ st, const cd *__restrict__ red,
+                            cd *__restrict__ Hall, double *__restrict__ nrm2, double *__restrict__ nav2) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= d.B) return;
+    const SlotState s = st[b];
+    const cd *r = red + (size_t)b * (BQ + 2);
+    if (s.phase == PH_RESTART && p == 0) {
+        nrm2[b] = r[BQ].x;
+        return;
+    }
+    const int nb = (s.i + BS) / BS;
+    if (s.phase != PH_ARNOLDI || p > nb) return;
+    if (p == 0) nav2[b] = r[BQ + 1].x;
+    if (p == nb) {
+        nrm2[b] = r[BQ].x;
+        return;
+    }
+    cd *H = Hall + (size_t)b * (d.m + 1) * d.m + (size_t)(d.m + 1) * s.i;
+    const int lo = p * BS, hi = min((p + 1) * BS, s.i + 1);
+    cd h[BS];
+    for (int j = 0; j < hi - lo; ++j) {
+        cd a = r[j];
+        for (int l = 0; l < j; ++l) a = csub(a, cmul(r[gram_idx(j, l)], h[l]));
+        h[j] = a;
+        H[lo + j] = a;
+    }
+}
+
 // ---- classical Gram-Schmidt with DGKS re-orthogonalisation (opts.ortho = 1) ----
 // Pass "dots": h_k = <v_k, w> for the 16 basis vectors k in [16 z, 16 z + 16) of
 // blockIdx.z (z = 0 also accumulates |w|^2 and turns restarting slots' W into
@@ -631,6 +778,20 @@ void launch_store_x(const KrylovDims &d, const int *col, const cd *V, cd *dst, l
     hipLaunchKernelGGL(k_store_x, dim3(g), dim3(256), 0, s, d, col, V, dst, ld);
     HIP_CHECK(hipGetLastError());
 }
+void launch_bmgs(const KrylovDims &d, cd *W, const cd *V, const cd *Rhs, const SlotState *st, cd *H, cd *pq,
+                 double *pnrm, double *pnav, cd *red, double *nrm2, double *nav2, int kmax, hipStream_t s) {
+    // passes 0 .. ceil((kmax+1)/BS) for the slot with the highest Arnoldi index kmax
+    const int np = (kmax + BS) / BS;
+    for (int p = 0; p <= np; ++p) {
+        hipLaunchKernelGGL(k_bmgs_pass, rs_grid(d), dim3(256), 0, s, d, p, W, V, Rhs, st, H, pq, pnrm, pnav);
+        hipLaunchKernelGGL(k_bmgs_reduce, dim3(BQ + 2, (unsigned)d.B), dim3(RED_T), 0, s, d, p, st, pq, pnrm, pnav,
+                           red);
+        hipLaunchKernelGGL(k_bmgs_coef, dim3((d.B + 63) / 64), dim3(64), 0, s, d, p, st, red, H, nrm2, nav2);
+    }
+    HIP_CHECK(hipGetLastError());
+}
+int bmgs_partials() { return BQ; }
+
 void launch_cgs(const KrylovDims &d, cd *W, const cd *V, const cd *Rhs, const SlotState *st, cd *H, cd *pdot,
                 double *pnrm, double *pnav, double *nrm2, double *nav2, int *flag, cd *cc, cd *hprev, int kmax,
                 hipStream_t s) {

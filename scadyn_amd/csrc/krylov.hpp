@@ -57,6 +57,13 @@ void launch_cgs(const KrylovDims &d, cd *W, const cd *V, const cd *Rhs, const Sl
                 double *pnrm, double *pnav, double *nrm2, double *nav2, int *flag, cd *cc, cd *hprev, int kmax,
                 hipStream_t s);
 
+// block MGS (MGS coefficients BS vectors per pass with in-pass Gram correction);
+// kmax = max over Arnoldi slots of i.  pq holds nwg*bmgs_partials()*B, red
+// B*(bmgs_partials()+2).
+void launch_bmgs(const KrylovDims &d, cd *W, const cd *V, const cd *Rhs, const SlotState *st, cd *H, cd *pq,
+                 double *pnrm, double *pnav, cd *red, double *nrm2, double *nav2, int kmax, hipStream_t s);
+int bmgs_partials();
+
 // matvec input offsets / activity derived on the device from the slot phases
 void launch_set_inputs(const KrylovDims &d, const SlotState *st, long long *in_off, int *active, hipStream_t s);
 // mark slots with col[b] >= 0 idle

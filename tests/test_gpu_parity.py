@@ -139,6 +139,30 @@ def test_solve_cgs_dgks():
     g.close()
 
 
+@pytest.mark.parametrize('maxit', [20, 50])
+def test_solve_block_mgs(maxit):
+    """Block MGS (opts.ortho = 2: 8 MGS coefficients per pass via the in-pass Gram
+    correction) against the one-pass-per-vector MGS on the device, at a tight
+    tolerance so that the cycles run through several blocks and restarts.  The
+    two differ only by rounding: iteration counts within one, solutions within
+    1e-8 (both converged to 1e-10)."""
+    from scadyn_amd import native
+    c, e, param, k = sphere_case(n_cells=6)
+    say(f'block MGS vs MGS, maxit {maxit}')
+    a = native.Operator(c, e, param, k, tol=1e-10, maxit=maxit, restart=6)
+    b = native.Operator(c, e, param, k, tol=1e-10, maxit=maxit, restart=6, ortho=2)
+    rng = np.random.default_rng(31)
+    n = 3 * e.shape[0]
+    rhs = rng.standard_normal((5, n)) + 1j * rng.standard_normal((5, n))
+    xa, ia = a.solve(rhs)
+    xb, ib = b.solve(rhs)
+    say(f'iterations {ia} vs {ib}')
+    assert np.max(np.abs(ia.astype(int) - ib.astype(int))) <= 1
+    assert max(rel(xb[r], xa[r]) for r in range(5)) < 1e-8
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize('nmax', [1, 3, 8])
 def test_project_mfma_gemm(nmax):
     """mat^H X on the FP64 matrix cores vs numpy, at 2nm = 6, 30 and 160 columns
