@@ -76,24 +76,26 @@ def build(verbose=False):
 
 
 SELFTEST = os.path.join(OBJDIR, 'fft_selftest')
+SELFTESTS = {'fft': ('fft_selftest.hip', SELFTEST), 'bmgs': ('bmgs_selftest.hip', os.path.join(OBJDIR, 'bmgs_selftest'))}
 
 
-def build_selftest(verbose=False):
-    """Host self-test of the register FFT stages (CPU only, AddressSanitizer on
-    the host code); links the library for the pass planner."""
+def build_selftest(verbose=False, which='fft'):
+    """Host self-tests of kernel code (CPU only, AddressSanitizer on the host
+    code): 'fft' the register FFT stages (links the library for the pass
+    planner), 'bmgs' the block Gram-Schmidt bodies."""
     hipcc = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
     os.makedirs(OBJDIR, exist_ok=True)
+    src, exe = SELFTESTS[which]
     cmd = [hipcc, '-O1', '-g', '-std=c++17', '--offload-arch=gfx950', '-Xarch_host', '-fsanitize=address',
            '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops', '-Wno-unused-command-line-argument',
-           '-o', SELFTEST, os.path.join(CSRC, 'fft_selftest.hip'), '-L' + HERE, '-lscadyn_amd',
-           '-Wl,-rpath,' + HERE]
+           '-o', exe, os.path.join(CSRC, src), '-L' + HERE, '-lscadyn_amd', '-Wl,-rpath,' + HERE]
     if verbose:
         print(' '.join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError('building the FFT self-test failed')
-    return SELFTEST
+        raise RuntimeError('building the ' + which + ' self-test failed')
+    return exe
 
 
 if __name__ == '__main__':

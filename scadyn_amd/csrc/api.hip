@@ -1252,6 +1252,12 @@ int scadyn_op_create(const scadyn_mesh *mesh, const scadyn_opts *opts, double k,
     if (o.maxit < 1 || o.restart < 1) fail(SCADYN_ERR_ARG, "maxit and restart must be >= 1");
     if (o.ortho < 0 || o.ortho > 2)
         fail(SCADYN_ERR_ARG, "ortho must be 0 (MGS), 1 (CGS + DGKS) or 2 (block MGS)");
+    // The first device run of the block-MGS path ended in a GPU memory fault whose
+    // cause is not found (its code passes the host self-test under
+    // AddressSanitizer); it stays disabled on the device until one is.
+    if (o.ortho == 2)
+        fail(SCADYN_ERR_UNSUPPORTED, "block MGS (ortho = 2) is disabled on the device: its first run faulted and "
+                                     "the cause is not yet found");
     if (o.fft_kernel != 0 && o.fft_kernel != 1) fail(SCADYN_ERR_ARG, "fft_kernel must be 0 (automatic) or 1 (Stockham)");
     *out = nullptr;
     std::unique_ptr<scadyn_operator> op(new scadyn_operator());

@@ -2,6 +2,7 @@
 #include <algorithm>
 
 #include "krylov.hpp"
+#include "bmgs_core.hpp"
 
 namespace scadyn {
 
@@ -143,19 +144,9 @@ __global__ __launch_bounds__(RED_T) void k_mgs_reduce(KrylovDims d, int k, const
 }
 
 // ---- block modified Gram-Schmidt (opts.ortho = 2) ------------------------------
-// The MGS recurrence h_k = <v_k, w - sum_{l<k} h_l v_l> evaluated BS basis vectors
-// per pass: pass p applies the deferred update w -= sum h_l v_l of block p-1 and
-// takes, from the same w, the dots d_j = <v_j, w> of block p together with the
-// block's Gram entries G_jl = <v_j, v_l> (l < j); then
-//   h_j = d_j - sum_{l<j in block} G_jl h_l
-// is exactly the MGS coefficient (forward substitution, k_bmgs_coef).  A pass
-// reads w once and 2 BS basis vectors, against 2 per coefficient for one MGS
-// pass per vector.  Pass nb = ceil((i+1)/BS) applies the last update and takes
-// |w|^2; pass 0 also takes |w|^2 before orthogonalisation (the reference's
-// re-orthogonalisation test) and turns restarting slots' W into rhs - A x.
-constexpr int BS = 8;
-constexpr int BQ = BS + BS * (BS - 1) / 2;  // dots + strictly lower Gram entries
-__device__ inline int gram_idx(int j, int l) { return BS + j * (j - 1) / 2 + l; }  // l < j
+// The per-thread / per-slot bodies are in bmgs_core.hpp (shared with the host
+// self-test, which runs them on the CPU under AddressSanitizer).
+using namespace bmgs;
 
 __global__ __launch_bounds__(256) void k_bmgs_pass(KrylovDims d, int p, cd *__restrict__ W,
                                                    const cd *__restrict__ V, const cd *__restrict__ Rhs,
@@ -164,78 +155,19 @@ __global__ __launch_bounds__(256) void k_bmgs_pass(KrylovDims d, int p, cd *__re
                                                    double *__restrict__ pnav) {
     __shared__ cd sq[256];
     __shared__ double sn1[256], sn2[256];
-    RowSlotMap m = row_slot_map(d.B);
+    const RowSlot m = row_slot(d.B, threadIdx.x, blockIdx.y);
     cd acc[BQ];
-#pragma unroll
-    for (int q = 0; q < BQ; ++q) acc[q] = cmk(0.0, 0.0);
     double nrm = 0.0, nav = 0.0;
-    if (m.ok) {
-        const SlotState s = st[m.b];
-        const size_t nB = (size_t)d.n * d.B;
-        const int nb = (s.i + BS) / BS;  // blocks of basis vectors 0..i
-        const bool rst = s.phase == PH_RESTART && p == 0;
-        const bool arn = s.phase == PH_ARNOLDI && p <= nb;
-        if (rst || arn) {
-            const int ulo = (p - 1) * BS, uhi = p >= 1 ? min(p * BS, s.i + 1) : 0;
-            const int dlo = p * BS, dhi = p < nb ? min((p + 1) * BS, s.i + 1) : 0;
-            cd h[BS];
-            const cd *Hc = H + (size_t)m.b * (d.m + 1) * d.m + (size_t)(d.m + 1) * s.i;
-#pragma unroll
-            for (int u = 0; u < BS; ++u) h[u] = (arn && ulo + u < uhi) ? Hc[ulo + u] : cmk(0.0, 0.0);
-            for (long long row = (long long)blockIdx.x * m.RW + m.rl; row < d.n; row += (long long)gridDim.x * m.RW) {
-                const size_t e = (size_t)row * d.B + m.b;
-                cd w = W[e];
-                if (rst) {
-                    w = csub(Rhs[e], w);
-                    W[e] = w;
-                    nrm += cabs2(w);
-                    continue;
-                }
-                if (p == 0) nav += cabs2(w);
-                if (p >= 1) {
-#pragma unroll
-                    for (int u = 0; u < BS; ++u)
-                        if (ulo + u < uhi) w = csub(w, cmul(h[u], V[(size_t)(ulo + u) * nB + e]));
-                    W[e] = w;
-                }
-                if (dhi > dlo) {
-                    cd v[BS];
-#pragma unroll
-                    for (int u = 0; u < BS; ++u) v[u] = (dlo + u < dhi) ? V[(size_t)(dlo + u) * nB + e] : cmk(0.0, 0.0);
-#pragma unroll
-                    for (int j = 0; j < BS; ++j) {
-                        acc[j] = cadd(acc[j], cmulc(v[j], w));
-#pragma unroll
-                        for (int l = 0; l < j; ++l) acc[gram_idx(j, l)] = cadd(acc[gram_idx(j, l)], cmulc(v[j], v[l]));
-                    }
-                } else {
-                    nrm += cabs2(w);
-                }
-            }
-        }
-    }
-    // workgroup partials: [blockIdx.x][q][B] (q < BQ), norms [blockIdx.x][B]
+    pass_rows(d, p, W, V, Rhs, st, H, m, blockIdx.x, gridDim.x, acc, nrm, nav);
     sn1[threadIdx.x] = nrm;
     sn2[threadIdx.x] = nav;
     __syncthreads();
-    if (m.ok && m.rl == 0) {
-        double a = 0, b = 0;
-        for (int r = 0; r < m.RW; ++r) {
-            a += sn1[r * m.SB + m.sl];
-            b += sn2[r * m.SB + m.sl];
-        }
-        pnrm[(size_t)blockIdx.x * d.B + m.b] = a;
-        pnav[(size_t)blockIdx.x * d.B + m.b] = b;
-    }
+    if (m.ok && m.rl == 0) store_norm_partials(d, m, blockIdx.x, sn1, sn2, pnrm, pnav);
     for (int q = 0; q < BQ; ++q) {
         __syncthreads();
         sq[threadIdx.x] = acc[q];
         __syncthreads();
-        if (m.ok && m.rl == 0) {
-            cd a = cmk(0.0, 0.0);
-            for (int r = 0; r < m.RW; ++r) a = cadd(a, sq[r * m.SB + m.sl]);
-            pq[((size_t)blockIdx.x * BQ + q) * d.B + m.b] = a;
-        }
+        if (m.ok && m.rl == 0) store_dot_partial(d, m, blockIdx.x, q, sq, pq);
     }
 }
 
@@ -244,49 +176,19 @@ __global__ __launch_bounds__(RED_T) void k_bmgs_reduce(KrylovDims d, int p, cons
                                                        const cd *__restrict__ pq, const double *__restrict__ pnrm,
                                                        const double *__restrict__ pnav, cd *__restrict__ red) {
     const int q = blockIdx.x, b = blockIdx.y;
-    const SlotState s = st[b];
-    const int nb = (s.i + BS) / BS;
-    const bool rst = s.phase == PH_RESTART && p == 0;
-    const bool arn = s.phase == PH_ARNOLDI && p <= nb;
-    if (!rst && !arn) return;
+    const ReduceSrc src = reduce_src(d, p, st, pq, pnrm, pnav, q, b);
+    if (src.kind == ReduceSrc::NONE) return;  // uniform over the workgroup
     cd a = cmk(0.0, 0.0);
-    if (q < BQ) {
-        if (arn && p < nb) a = block_sum<cd>(pq, d.nwg, (size_t)BQ * d.B, (size_t)q * d.B + b);
-    } else if (q == BQ) {
-        a.x = block_sum<double>(pnrm, d.nwg, d.B, b);
-    } else {
-        if (arn && p == 0) a.x = block_sum<double>(pnav, d.nwg, d.B, b);
-    }
-    if (threadIdx.x == 0) red[(size_t)b * (BQ + 2) + q] = a;
+    if (src.kind == ReduceSrc::DOT) a = block_sum<cd>(src.z, d.nwg, src.stride, src.off);
+    if (src.kind == ReduceSrc::NORM) a.x = block_sum<double>(src.r, d.nwg, src.stride, src.off);
+    if (threadIdx.x == 0) red[red_index(q, b)] = a;
 }
 
-// forward substitution h_j = d_j - sum_{l<j} G_jl h_l into H column i; norms
 __global__ void k_bmgs_coef(KrylovDims d, int p, const SlotState *__restrict__ st, const cd *__restrict__ red,
                             cd *__restrict__ Hall, double *__restrict__ nrm2, double *__restrict__ nav2) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= d.B) return;
-    const SlotState s = st[b];
-    const cd *r = red + (size_t)b * (BQ + 2);
-    if (s.phase == PH_RESTART && p == 0) {
-        nrm2[b] = r[BQ].x;
-        return;
-    }
-    const int nb = (s.i + BS) / BS;
-    if (s.phase != PH_ARNOLDI || p > nb) return;
-    if (p == 0) nav2[b] = r[BQ + 1].x;
-    if (p == nb) {
-        nrm2[b] = r[BQ].x;
-        return;
-    }
-    cd *H = Hall + (size_t)b * (d.m + 1) * d.m + (size_t)(d.m + 1) * s.i;
-    const int lo = p * BS, hi = min((p + 1) * BS, s.i + 1);
-    cd h[BS];
-    for (int j = 0; j < hi - lo; ++j) {
-        cd a = r[j];
-        for (int l = 0; l < j; ++l) a = csub(a, cmul(r[gram_idx(j, l)], h[l]));
-        h[j] = a;
-        H[lo + j] = a;
-    }
+    coef_slot(d, p, st, red, Hall, nrm2, nav2, b);
 }
 
 // ---- classical Gram-Schmidt with DGKS re-orthogonalisation (opts.ortho = 1) ----
