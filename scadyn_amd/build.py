@@ -76,13 +76,15 @@ def build(verbose=False):
 
 
 SELFTEST = os.path.join(OBJDIR, 'fft_selftest')
-SELFTESTS = {'fft': ('fft_selftest.hip', SELFTEST), 'bmgs': ('bmgs_selftest.hip', os.path.join(OBJDIR, 'bmgs_selftest'))}
+SELFTESTS = {'fft': ('fft_selftest.hip', SELFTEST), 'bmgs': ('bmgs_selftest.hip', os.path.join(OBJDIR, 'bmgs_selftest')),
+             'gather': ('gather_selftest.hip', os.path.join(OBJDIR, 'gather_selftest'))}
 
 
 def build_selftest(verbose=False, which='fft'):
     """Host self-tests of kernel code (CPU only, AddressSanitizer on the host
     code): 'fft' the register FFT stages (links the library for the pass
-    planner), 'bmgs' the block Gram-Schmidt bodies."""
+    planner), 'bmgs' the block Gram-Schmidt bodies, 'gather' the LDS-staged
+    per-cube gather."""
     hipcc = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
     os.makedirs(OBJDIR, exist_ok=True)
     src, exe = SELFTESTS[which]

@@ -1,0 +1,201 @@
+// Gather + near zone with LDS-staged operands, one wave per (non-empty cube,
+// 64-slot chunk): the per-lane steps, host+device so that the host self-test
+// runs the same code on the CPU.
+//
+//  * grid part: the S* records of up to GT tets of the cube (contiguous, 108
+//    complex each) are copied to the staging buffer with coalesced loads, then
+//    read as wave-uniform broadcasts while each lane streams its slot's 27 box
+//    nodes (gather, src/sparse.f90:47-77, and Y2 - k^2 Y,
+//    src/gmres_module.f90:148-153);
+//  * near part (matmul_Acorr_const, src/sparse_mat.f90:96-123): for each near
+//    cube, in chunks of VC column tets, lane (u, v) = (lane / VC, lane % VC)
+//    fetches the block of row tet u and column tet v, converts it once and
+//    stores it as a dense 3x3 (transposed when the pair is stored as (v, u));
+//    every lane then applies the chunk's blocks to its slot's x of those tets.
+// The rows of a cube's tets list the near tets in the same order (near cube
+// offset, then tet id), so entry q of every row of the cube is column q.
+//
+// Buffers: grid [Nz][Ny][Nx][4][B], x and y [3N][B] (x at in_off[b]), S4
+// [N][27][4], row_start [N+1], ent_blk [row_start[N]], blocks [P][9] complex64.
+#pragma once
+
+#include "internal.hpp"
+
+namespace scadyn {
+namespace gcore {
+
+#define GC_HD __host__ __device__ inline
+
+template <typename C>
+struct GatherArgs {
+    const C *grid;
+    const cd *xin;
+    const long long *in_off;
+    int B;
+    const int *ne_cubes;
+    int n_ne;
+    const int *cube_start;
+    const C *S4;
+    int Nx, Ny, ncx, ncy, ncz;
+    const int64_t *row_start;
+    const uint32_t *ent_blk;
+    const cf *blocks;
+    double k2;
+    const int *slot_active;
+    cd *y;
+};
+
+template <int GT, int VC>
+struct Shape {
+    static constexpr int LDS_N = (GT * 108 > GT * VC * 9) ? GT * 108 : GT * VC * 9;  // staging elements
+};
+
+// one wave's item: cube and slot
+struct Item {
+    int c, cx, cy, cz, t0, t1;
+    int b;       // slot used for loads (B - 1 for idle lanes of a partial chunk)
+    bool valid;  // lane holds a real slot
+    bool act;    // slot active (inactive slots get y = 0)
+    size_t base; // first box node of the cube
+};
+
+template <typename C>
+GC_HD Item make_item(const GatherArgs<C> &g, long long w, int lane) {
+    const int nch = (g.B + 63) >> 6;
+    const int ci = (int)(w / nch);
+    const int braw = (int)(w - (long long)ci * nch) * 64 + lane;
+    Item it;
+    it.valid = braw < g.B;
+    it.b = it.valid ? braw : g.B - 1;
+    it.c = g.ne_cubes[ci];
+    it.cx = it.c % g.ncx;
+    it.cy = (it.c / g.ncx) % g.ncy;
+    it.cz = it.c / (g.ncx * g.ncy);
+    it.t0 = g.cube_start[it.c];
+    it.t1 = g.cube_start[it.c + 1];
+    it.act = g.slot_active ? g.slot_active[it.b] != 0 : true;
+    it.base = (size_t)(2 * it.cx) + (size_t)g.Nx * ((size_t)(2 * it.cy) + (size_t)g.Ny * (2 * it.cz));
+    return it;
+}
+
+// stage the S* records of tets tp .. tp+nt-1 (lane's share)
+template <typename C>
+GC_HD void stage_s4(const GatherArgs<C> &g, int tp, int nt, int lane, C *sm) {
+    for (int e = lane; e < nt * 108; e += 64) sm[e] = g.S4[(size_t)tp * 108 + e];
+}
+
+template <typename C, int GT>
+GC_HD void zero_acc(C (*acc)[3]) {
+#pragma unroll
+    for (int u = 0; u < GT; ++u) acc[u][0] = acc[u][1] = acc[u][2] = Prec<C>::zero();
+}
+
+// y_c += S_{c+1}^T g3 - k^2 S^T g_c over the 27 box nodes
+template <typename C, int GT>
+GC_HD void grid_part(const GatherArgs<C> &g, const Item &it, int nt, const C *sm, C (*acc)[3]) {
+    typedef typename Prec<C>::real R;
+    const R kk = (R)g.k2;
+    const int B = g.B;
+    for (int jz = 0; jz < 3; ++jz)
+        for (int jy = 0; jy < 3; ++jy)
+            for (int jx = 0; jx < 3; ++jx) {
+                const int j = jx + 3 * jy + 9 * jz;
+                const C *gp = g.grid + (it.base + jx + (size_t)g.Nx * (jy + (size_t)g.Ny * jz)) * 4 * B + it.b;
+                const C g0 = gp[0], g1 = gp[(size_t)B], g2 = gp[(size_t)2 * B], g3 = gp[(size_t)3 * B];
+#pragma unroll
+                for (int u = 0; u < GT; ++u) {
+                    if (u >= nt) break;
+                    const C *s = sm + u * 108 + j * 4;
+                    const C s0 = cscale(s[0], -kk);
+                    cfma(acc[u][0], s0, g0);
+                    cfma(acc[u][1], s0, g1);
+                    cfma(acc[u][2], s0, g2);
+                    cfma(acc[u][0], s[1], g3);
+                    cfma(acc[u][1], s[2], g3);
+                    cfma(acc[u][2], s[3], g3);
+                }
+            }
+}
+
+// lane's block of a near chunk: row tet tp + lane / VC, column v0 + lane % VC,
+// entry q + lane % VC of the row (rsu = row_start of the lane's row tet)
+template <typename C, int GT, int VC>
+GC_HD void stage_blocks(const GatherArgs<C> &g, int nt, int nv, int q, int lane, int64_t rsu, C *sm) {
+    const int su = lane / VC, sv = lane % VC;
+    if (su >= nt || sv >= nv) return;
+    const uint32_t eb = g.ent_blk[rsu + q + sv];
+    const cf *a = g.blocks + (size_t)(eb & 0x7fffffffu) * 9;
+    C *dst = sm + (su * VC + sv) * 9;
+    if (eb & 0x80000000u) {
+        for (int i = 0; i < 3; ++i)
+            for (int jj = 0; jj < 3; ++jj) dst[i + 3 * jj] = Prec<C>::from(a[jj + 3 * i]);
+    } else {
+        for (int i = 0; i < 9; ++i) dst[i] = Prec<C>::from(a[i]);
+    }
+}
+
+template <typename C, int GT, int VC>
+GC_HD int64_t lane_row_start(const GatherArgs<C> &g, int tp, int nt, int lane) {
+    const int su = lane / VC;
+    return su < nt ? g.row_start[tp + su] : 0;
+}
+
+// apply the staged blocks of column tets v0 .. v0+nv-1
+template <typename C, int GT, int VC>
+GC_HD void chunk_part(const GatherArgs<C> &g, const Item &it, int nt, int v0, int nv, const C *sm, C (*acc)[3]) {
+    const int B = g.B;
+    const cd *xb = g.xin + g.in_off[it.b] + it.b;
+    for (int v = 0; v < nv; ++v) {
+        const int col = v0 + v;
+        const C x0 = Prec<C>::from(xb[(size_t)(3 * col) * B]), x1 = Prec<C>::from(xb[(size_t)(3 * col + 1) * B]),
+                x2 = Prec<C>::from(xb[(size_t)(3 * col + 2) * B]);
+#pragma unroll
+        for (int u = 0; u < GT; ++u) {
+            if (u >= nt) break;
+            const C *m = sm + (u * VC + v) * 9;
+            cfma(acc[u][0], m[0], x0);
+            cfma(acc[u][0], m[3], x1);
+            cfma(acc[u][0], m[6], x2);
+            cfma(acc[u][1], m[1], x0);
+            cfma(acc[u][1], m[4], x1);
+            cfma(acc[u][1], m[7], x2);
+            cfma(acc[u][2], m[2], x0);
+            cfma(acc[u][2], m[5], x1);
+            cfma(acc[u][2], m[8], x2);
+        }
+    }
+}
+
+template <typename C, int GT>
+GC_HD void write_out(const GatherArgs<C> &g, const Item &it, int tp, int nt, C (*acc)[3]) {
+    if (!it.valid) return;
+#pragma unroll
+    for (int u = 0; u < GT; ++u) {
+        if (u >= nt) break;
+        cd *yo = g.y + (size_t)(3 * (tp + u)) * g.B + it.b;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) yo[(size_t)r * g.B] = it.act ? Prec<cd>::from(acc[u][r]) : cmk(0, 0);
+    }
+}
+
+// the near cubes of (cx, cy, cz) in row order; calls f(first tet, end tet)
+template <typename C, typename F>
+GC_HD void for_near_cubes(const GatherArgs<C> &g, const Item &it, F &&f) {
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int zz = it.cz + dz;
+        if (zz < 0 || zz >= g.ncz) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int yy = it.cy + dy;
+            if (yy < 0 || yy >= g.ncy) continue;
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int xx = it.cx + dx;
+                if (xx < 0 || xx >= g.ncx) continue;
+                const int nc = xx + g.ncx * (yy + g.ncy * zz);
+                f(g.cube_start[nc], g.cube_start[nc + 1]);
+            }
+        }
+    }
+}
+
+}  // namespace gcore
+}  // namespace scadyn

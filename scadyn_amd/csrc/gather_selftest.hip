@@ -1,0 +1,172 @@
+// Host self-test of the LDS-staged per-cube gather (gather_core.hpp).
+//
+// Runs the kernel's own per-lane steps on the CPU, one wave at a time, with
+// the workgroup barriers as phase boundaries, over exact-size buffers
+// (AddressSanitizer on the host build reports any access outside them), and
+// compares every output with a direct evaluation of
+//   y_t = sum_j (S_{c+1}[t][j] g3[node_j] - k^2 S[t][j] g_c[node_j])
+//       + sum_{near tets v} A(t, v) x_v
+// on a synthetic cube lattice: 0 to 11 tets per cube (empty cubes, cubes with
+// more tets than one staging pass holds), near-zone entries pointing at
+// shuffled blocks with and without the transpose flag, inactive slots, and
+// batch widths 64, 112 (partial second chunk) and 56.
+//
+// Exit status 0 when every case agrees to 1e-13 (complex128) / 1e-5 (complex64).
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <random>
+#include <vector>
+
+#include "gather_core.hpp"
+
+using namespace scadyn;
+using namespace scadyn::gcore;
+
+static int failures = 0;
+
+template <typename C>
+static C to(cd a) {
+    return Prec<C>::from(a);
+}
+
+template <typename C, int GT, int VC>
+static void emulate(const GatherArgs<C> &g) {
+    std::vector<C> sm(Shape<GT, VC>::LDS_N);
+    std::vector<std::array<std::array<C, 3>, GT>> acc(64);
+    const long long nw = (long long)g.n_ne * ((g.B + 63) >> 6);
+    for (long long w = 0; w < nw; ++w) {
+        std::vector<Item> it(64);
+        for (int l = 0; l < 64; ++l) it[l] = make_item(g, w, l);
+        for (int tp = it[0].t0; tp < it[0].t1; tp += GT) {
+            const int nt = std::min(GT, it[0].t1 - tp);
+            for (int l = 0; l < 64; ++l) zero_acc<C, GT>((C(*)[3])acc[l].data());
+            for (int l = 0; l < 64; ++l) stage_s4(g, tp, nt, l, sm.data());
+            for (int l = 0; l < 64; ++l) grid_part<C, GT>(g, it[l], nt, sm.data(), (C(*)[3])acc[l].data());
+            std::vector<int64_t> rsu(64);
+            for (int l = 0; l < 64; ++l) rsu[l] = lane_row_start<C, GT, VC>(g, tp, nt, l);
+            int q = 0;
+            for_near_cubes(g, it[0], [&](int c0, int c1) {
+                for (int v0 = c0; v0 < c1; v0 += VC) {
+                    const int nv = std::min(VC, c1 - v0);
+                    for (int l = 0; l < 64; ++l) stage_blocks<C, GT, VC>(g, nt, nv, q, l, rsu[l], sm.data());
+                    for (int l = 0; l < 64; ++l)
+                        chunk_part<C, GT, VC>(g, it[l], nt, v0, nv, sm.data(), (C(*)[3])acc[l].data());
+                    q += nv;
+                }
+            });
+            for (int l = 0; l < 64; ++l) write_out<C, GT>(g, it[l], tp, nt, (C(*)[3])acc[l].data());
+        }
+    }
+}
+
+template <typename C>
+static void run_case(int B, std::mt19937_64 &rng) {
+    const int ncx = 4, ncy = 3, ncz = 3;
+    const int Nx = 2 * ncx + 2, Ny = 2 * ncy + 2, Nz = 2 * ncz + 2;
+    const int ncubes = ncx * ncy * ncz;
+    std::uniform_int_distribution<int> cnt(0, 11);
+    std::vector<int> cube_start(ncubes + 1, 0);
+    for (int c = 0; c < ncubes; ++c) cube_start[c + 1] = cube_start[c] + (c % 5 == 0 ? 0 : cnt(rng));
+    const int N = cube_start[ncubes];
+    std::vector<int> tet_cube(N), ne;
+    for (int c = 0; c < ncubes; ++c) {
+        for (int t = cube_start[c]; t < cube_start[c + 1]; ++t) tet_cube[t] = c;
+        if (cube_start[c + 1] > cube_start[c]) ne.push_back(c);
+    }
+    // rows in near-cube order; each (t, v) refers to a block, transposed for half of them
+    std::vector<int64_t> row_start(N + 1, 0);
+    std::vector<int> ent_col;
+    for (int t = 0; t < N; ++t) {
+        const int c = tet_cube[t], cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int xx = cx + dx, yy = cy + dy, zz = cz + dz;
+                    if (xx < 0 || xx >= ncx || yy < 0 || yy >= ncy || zz < 0 || zz >= ncz) continue;
+                    const int nc = xx + ncx * (yy + ncy * zz);
+                    for (int v = cube_start[nc]; v < cube_start[nc + 1]; ++v) ent_col.push_back(v);
+                }
+        row_start[t + 1] = (int64_t)ent_col.size();
+    }
+    const size_t E = ent_col.size();
+    const size_t P = E / 2 + 1;
+    std::normal_distribution<double> nd;
+    std::uniform_int_distribution<size_t> pick(0, P - 1);
+    std::vector<uint32_t> ent_blk(E);
+    for (size_t e = 0; e < E; ++e) ent_blk[e] = (uint32_t)pick(rng) | ((rng() & 1) ? 0x80000000u : 0u);
+    std::vector<cf> blocks(P * 9);
+    for (auto &z : blocks) z = cmkf((float)nd(rng), (float)nd(rng));
+    std::vector<C> S4((size_t)N * 108), grid((size_t)Nx * Ny * Nz * 4 * B);
+    for (auto &z : S4) z = to<C>(cmk(nd(rng), nd(rng)));
+    for (auto &z : grid) z = to<C>(cmk(nd(rng), nd(rng)));
+    // x: slot b's vector at in_off[b] + row*B + b inside a buffer of two vector blocks
+    std::vector<cd> xin((size_t)2 * 3 * N * B);
+    for (auto &z : xin) z = cmk(nd(rng), nd(rng));
+    std::vector<long long> in_off(B);
+    std::vector<int> active(B);
+    for (int b = 0; b < B; ++b) {
+        in_off[b] = (b % 3 == 1) ? (long long)3 * N * B : 0;
+        active[b] = (b % 7 != 3);
+    }
+    std::vector<cd> y((size_t)3 * N * B, cmk(-7.0, -7.0));
+    const double k2 = 1.7;
+    GatherArgs<C> g = {grid.data(), xin.data(), in_off.data(), B, ne.data(), (int)ne.size(), cube_start.data(),
+                       S4.data(), Nx, Ny, ncx, ncy, ncz, row_start.data(), ent_blk.data(), blocks.data(), k2,
+                       active.data(), y.data()};
+    emulate<C, 4, 16>(g);
+    // direct evaluation in double
+    double num = 0.0, den = 0.0;
+    for (int t = 0; t < N; ++t) {
+        const int c = tet_cube[t], cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
+        for (int b = 0; b < B; ++b) {
+            cd acc[3] = {cmk(0, 0), cmk(0, 0), cmk(0, 0)};
+            if (active[b]) {
+                for (int j = 0; j < 27; ++j) {
+                    const int jx = j % 3, jy = (j / 3) % 3, jz = j / 9;
+                    const size_t node = (size_t)(2 * cx + jx) + (size_t)Nx * ((2 * cy + jy) + (size_t)Ny * (2 * cz + jz));
+                    for (int r = 0; r < 3; ++r) {
+                        const cd gr = Prec<cd>::from(grid[(node * 4 + r) * B + b]);
+                        const cd g3 = Prec<cd>::from(grid[(node * 4 + 3) * B + b]);
+                        const cd s0 = Prec<cd>::from(S4[((size_t)t * 27 + j) * 4]);
+                        const cd sr = Prec<cd>::from(S4[((size_t)t * 27 + j) * 4 + 1 + r]);
+                        acc[r] = cadd(acc[r], csub(cmul(sr, g3), cscale(cmul(s0, gr), k2)));
+                    }
+                }
+                for (int64_t e = row_start[t]; e < row_start[t + 1]; ++e) {
+                    const int v = ent_col[e];
+                    const uint32_t eb = ent_blk[e];
+                    const cf *a = &blocks[(size_t)(eb & 0x7fffffffu) * 9];
+                    for (int i = 0; i < 3; ++i)
+                        for (int jj = 0; jj < 3; ++jj) {
+                            const cf m = (eb & 0x80000000u) ? a[jj + 3 * i] : a[i + 3 * jj];
+                            const cd xv = xin[in_off[b] + (size_t)(3 * v + jj) * B + b];
+                            acc[i] = cadd(acc[i], cmul(Prec<cd>::from(m), xv));
+                        }
+                }
+            }
+            for (int r = 0; r < 3; ++r) {
+                const cd got = y[(size_t)(3 * t + r) * B + b];
+                num += cabs2(csub(got, acc[r]));
+                den += cabs2(acc[r]) + 1e-300;
+            }
+        }
+    }
+    const double err = std::sqrt(num / den);
+    const double tol = sizeof(C) == 16 ? 1e-13 : 1e-5;
+    const bool ok = err < tol;
+    if (!ok) ++failures;
+    std::printf("%s B=%3d N=%4d cubes %d (non-empty %zu) entries %zu  rel.err %.2e %s\n",
+                sizeof(C) == 16 ? "c128" : "c64 ", B, N, ncubes, ne.size(), E, err, ok ? "ok" : "FAIL");
+}
+
+int main() {
+    std::mt19937_64 rng(99);
+    for (int B : {64, 112, 56}) {
+        run_case<cd>(B, rng);
+        run_case<cf>(B, rng);
+    }
+    std::printf("%d failures\n", failures);
+    return failures ? 1 : 0;
+}

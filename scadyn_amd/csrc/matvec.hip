@@ -7,6 +7,7 @@
 // Sz^T x_z; by linearity this one convolution replaces the reference's three
 // divergence convolutions (:122-144), whose gathered sum it equals.
 #include "matvec.hpp"
+#include "gather_core.hpp"
 
 namespace scadyn {
 
@@ -460,134 +461,40 @@ __global__ __launch_bounds__(256) void k_gather_cube_w64(
     }
 }
 
-// Gather + near zone with the per-tet operands staged in LDS: one wave (one
-// 64-thread workgroup) per (non-empty cube, 64-slot chunk).
-//  * grid part: the S* records of up to GT tets of the cube (contiguous, 108
-//    complex each) are copied to LDS with coalesced loads, then read as
-//    wave-uniform broadcasts while each lane streams its slot's 27 box nodes;
-//  * near part: for each near cube, in chunks of VC column tets, lane
-//    (u, v) = (lane / VC, lane % VC) fetches the block of row tet u and column
-//    tet v, converts it once and stores it (transposed when the pair is stored
-//    as (v, u)) as a dense 3x3 in LDS; every lane then applies the chunk's
-//    blocks to its slot's x of those column tets.
-// Compared with scalar (SMEM) operand loads this keeps many operand fetches in
-// flight and converts each block once per wave instead of once per lane.
+// Gather + near zone with the per-tet operands staged in LDS, one 64-thread
+// workgroup per (non-empty cube, 64-slot chunk); the steps are in
+// gather_core.hpp (shared with the host self-test).  Compared with scalar
+// (SMEM) operand loads this keeps many operand fetches in flight and converts
+// each near-zone block once per wave instead of once per lane.
 template <typename C, int GT, int VC>
-__global__ __launch_bounds__(64, 3) void k_gather_lds(
-    const C *__restrict__ grid, const cd *__restrict__ xin, const long long *__restrict__ in_off, int B,
-    const int *__restrict__ ne_cubes, int n_ne, const int *__restrict__ cube_start, const C *__restrict__ S4,
-    int Nx, int Ny, int ncx, int ncy, int ncz, const int64_t *__restrict__ row_start,
-    const uint32_t *__restrict__ ent_blk, const cf *__restrict__ blocks, double k2,
-    const int *__restrict__ slot_active, cd *__restrict__ y) {
-    static_assert(GT * VC == 64, "one block per lane when staging");
-    typedef typename Prec<C>::real R;
-    constexpr int LDS_N = (GT * 108 > GT * VC * 9) ? GT * 108 : GT * VC * 9;
-    __shared__ C sm[LDS_N];
+__global__ __launch_bounds__(64, 2) void k_gather_lds(gcore::GatherArgs<C> g) {
+    using namespace gcore;
+    __shared__ C sm[Shape<GT, VC>::LDS_N];
     const int lane = threadIdx.x;
-    const int nch = (B + 63) >> 6;
-    const long long nw = (long long)n_ne * nch;
-    const R kk = (R)k2;
+    const long long nw = (long long)g.n_ne * ((g.B + 63) >> 6);
     for (long long w = blockIdx.x; w < nw; w += gridDim.x) {
-        const int ci = (int)(w / nch);
-        const int braw = (int)(w - (long long)ci * nch) * 64 + lane;
-        const bool valid = braw < B;
-        const int b = valid ? braw : B - 1;
-        const int c = ne_cubes[ci];
-        const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
-        const int t0 = cube_start[c], t1 = cube_start[c + 1];
-        const bool act = slot_active ? slot_active[b] != 0 : true;
-        const cd *xb = xin + in_off[b] + b;
-        const size_t base = (size_t)(2 * cx) + (size_t)Nx * ((size_t)(2 * cy) + (size_t)Ny * (2 * cz));
-        for (int tp = t0; tp < t1; tp += GT) {
-            const int nt = min(GT, t1 - tp);
+        const Item it = make_item(g, w, lane);
+        for (int tp = it.t0; tp < it.t1; tp += GT) {
+            const int nt = min(GT, it.t1 - tp);
             C acc[GT][3];
-#pragma unroll
-            for (int u = 0; u < GT; ++u) acc[u][0] = acc[u][1] = acc[u][2] = Prec<C>::zero();
-            // ---- grid part
+            zero_acc<C, GT>(acc);
             __syncthreads();
-#pragma unroll 1
-            for (int e = lane; e < nt * 108; e += 64) sm[e] = S4[(size_t)tp * 108 + e];
+            stage_s4(g, tp, nt, lane, sm);
             __syncthreads();
-            for (int jz = 0; jz < 3; ++jz)
-                for (int jy = 0; jy < 3; ++jy)
-#pragma unroll
-                    for (int jx = 0; jx < 3; ++jx) {
-                        const int j = jx + 3 * jy + 9 * jz;
-                        const C *g = grid + (base + jx + (size_t)Nx * (jy + (size_t)Ny * jz)) * 4 * B + b;
-                        const C g0 = g[0], g1 = g[(size_t)B], g2 = g[(size_t)2 * B], g3 = g[(size_t)3 * B];
-#pragma unroll
-                        for (int u = 0; u < GT; ++u) {
-                            if (u >= nt) break;
-                            const C *s = sm + u * 108 + j * 4;
-                            const C s0 = cscale(s[0], -kk);
-                            cfma(acc[u][0], s0, g0);
-                            cfma(acc[u][1], s0, g1);
-                            cfma(acc[u][2], s0, g2);
-                            cfma(acc[u][0], s[1], g3);
-                            cfma(acc[u][1], s[2], g3);
-                            cfma(acc[u][2], s[3], g3);
-                        }
-                    }
-            // ---- near part
-            const int su = lane / VC, sv = lane % VC;
-            const int64_t rsu = su < nt ? row_start[tp + su] : 0;
+            grid_part<C, GT>(g, it, nt, sm, acc);
+            const int64_t rsu = lane_row_start<C, GT, VC>(g, tp, nt, lane);
             int q = 0;
-            for (int dz = -1; dz <= 1; ++dz) {
-                const int zz = cz + dz;
-                if (zz < 0 || zz >= ncz) continue;
-                for (int dy = -1; dy <= 1; ++dy) {
-                    const int yy = cy + dy;
-                    if (yy < 0 || yy >= ncy) continue;
-                    for (int dx = -1; dx <= 1; ++dx) {
-                        const int xx = cx + dx;
-                        if (xx < 0 || xx >= ncx) continue;
-                        const int nc = xx + ncx * (yy + ncy * zz);
-                        const int c0 = cube_start[nc], c1 = cube_start[nc + 1];
-                        for (int v0 = c0; v0 < c1; v0 += VC) {
-                            const int nv = min(VC, c1 - v0);
-                            __syncthreads();
-                            if (su < nt && sv < nv) {
-                                const uint32_t eb = ent_blk[rsu + q + sv];
-                                const cf *a = blocks + (size_t)(eb & 0x7fffffffu) * 9;
-                                C *dst = sm + (su * VC + sv) * 9;
-                                if (eb & 0x80000000u) {
-#pragma unroll
-                                    for (int i = 0; i < 3; ++i)
-#pragma unroll
-                                        for (int jj = 0; jj < 3; ++jj) dst[i + 3 * jj] = Prec<C>::from(a[jj + 3 * i]);
-                                } else {
-#pragma unroll
-                                    for (int i = 0; i < 9; ++i) dst[i] = Prec<C>::from(a[i]);
-                                }
-                            }
-                            __syncthreads();
-#pragma unroll 2
-                            for (int v = 0; v < nv; ++v) {
-                                const int col = v0 + v;
-                                const C x0 = Prec<C>::from(xb[(size_t)(3 * col) * B]),
-                                        x1 = Prec<C>::from(xb[(size_t)(3 * col + 1) * B]),
-                                        x2 = Prec<C>::from(xb[(size_t)(3 * col + 2) * B]);
-#pragma unroll
-                                for (int u = 0; u < GT; ++u) {
-                                    if (u >= nt) break;
-                                    const C *m = sm + (u * VC + v) * 9;
-                                    cfma(acc[u][0], m[0], x0); cfma(acc[u][0], m[3], x1); cfma(acc[u][0], m[6], x2);
-                                    cfma(acc[u][1], m[1], x0); cfma(acc[u][1], m[4], x1); cfma(acc[u][1], m[7], x2);
-                                    cfma(acc[u][2], m[2], x0); cfma(acc[u][2], m[5], x1); cfma(acc[u][2], m[8], x2);
-                                }
-                            }
-                            q += nv;
-                        }
-                    }
+            for_near_cubes(g, it, [&](int c0, int c1) {
+                for (int v0 = c0; v0 < c1; v0 += VC) {
+                    const int nv = min(VC, c1 - v0);
+                    __syncthreads();
+                    stage_blocks<C, GT, VC>(g, nt, nv, q, lane, rsu, sm);
+                    __syncthreads();
+                    chunk_part<C, GT, VC>(g, it, nt, v0, nv, sm, acc);
+                    q += nv;
                 }
-            }
-#pragma unroll
-            for (int u = 0; u < GT; ++u) {
-                if (u >= nt || !valid) break;
-                cd *yo = y + (size_t)(3 * (tp + u)) * B + b;
-#pragma unroll
-                for (int r = 0; r < 3; ++r) yo[(size_t)r * B] = act ? Prec<cd>::from(acc[u][r]) : cmk(0, 0);
-            }
+            });
+            write_out<C, GT>(g, it, tp, nt, acc);
         }
     }
 }
@@ -620,9 +527,11 @@ void launch_gather_near(const MatvecArgs<C> &a, hipStream_t s) {
     if (wave_slots(a.B) && a.ne_cubes) {
         const long long nw = (long long)a.n_ne * ((a.B + 63) / 64);
         if (a.gather_lds) {
-            hipLaunchKernelGGL((k_gather_lds<C, 4, 16>), dim3(grid_for(nw, 1)), dim3(64), 0, s, a.grid_out, a.xin,
-                               a.in_off, a.B, a.ne_cubes, a.n_ne, a.cube_start, a.S4, a.Nx, a.Ny, a.ncx, a.ncy,
-                               a.ncz, a.row_start, a.ent_blk, a.blocks, a.k2, a.slot_active, a.y);
+            const gcore::GatherArgs<C> g = {a.grid_out, a.xin,     a.in_off,    a.B,      a.ne_cubes,
+                                            a.n_ne,     a.cube_start, a.S4,      a.Nx,     a.Ny,
+                                            a.ncx,      a.ncy,       a.ncz,       a.row_start, a.ent_blk,
+                                            a.blocks,   a.k2,        a.slot_active, a.y};
+            hipLaunchKernelGGL((k_gather_lds<C, 4, 16>), dim3(grid_for(nw, 1)), dim3(64), 0, s, g);
         } else {
             hipLaunchKernelGGL((k_gather_cube_w64<C, 8>), dim3(grid_for(nw, 4)), dim3(256), 0, s, a.grid_out, a.xin,
                                a.in_off, a.B, a.ne_cubes, a.n_ne, a.cube_start, a.S4, a.Nx, a.Ny, a.ncx, a.ncy,
