@@ -5,7 +5,7 @@
 namespace scadyn {
 namespace regfft {
 
-template <int MODE, typename C>
+template <int MODE, typename C, int RMAX>
 __global__ __launch_bounds__(512) void k_line_reg(const C *__restrict__ in, C *__restrict__ out, LinePass p,
                                                   const C *__restrict__ twg, int sign, const C *__restrict__ fg,
                                                   ConvInfo ci) {
@@ -14,15 +14,15 @@ __global__ __launch_bounds__(512) void k_line_reg(const C *__restrict__ in, C *_
     C *lds = tw + p.L;
     for (int m = threadIdx.x; m < p.L; m += blockDim.x) tw[m] = twg[m];
     const Ctx x = make_ctx(p, blockIdx.x, threadIdx.x);
-    C v[16];
-    line_reg_phase<MODE, C>(0, in, out, lds, tw, fg, ci, p, x, sign, v);
+    C v[RMAX];
+    line_reg_phase<MODE, C, RMAX>(0, in, out, lds, tw, fg, ci, p, x, sign, v);
     __syncthreads();
-    line_reg_phase<MODE, C>(1, in, out, lds, tw, fg, ci, p, x, sign, v);
+    line_reg_phase<MODE, C, RMAX>(1, in, out, lds, tw, fg, ci, p, x, sign, v);
     if (MODE == 1) {
         __syncthreads();  // every stage-2 read of LDS is done
-        line_reg_phase<MODE, C>(2, in, out, lds, tw, fg, ci, p, x, sign, v);
+        line_reg_phase<MODE, C, RMAX>(2, in, out, lds, tw, fg, ci, p, x, sign, v);
         __syncthreads();
-        line_reg_phase<MODE, C>(3, in, out, lds, tw, fg, ci, p, x, sign, v);
+        line_reg_phase<MODE, C, RMAX>(3, in, out, lds, tw, fg, ci, p, x, sign, v);
     }
 }
 
@@ -48,12 +48,19 @@ void launch_line_reg(const LinePass &p, const C *in, C *out, const C *tw, int si
     if (blocks <= 0) return;
     if (blocks > 0x7fffffffLL) fail(SCADYN_ERR_UNSUPPORTED, "FFT grid too large");
     const size_t lds = line_pass_lds(p, sizeof(C));
-    if (mode == 0)
-        hipLaunchKernelGGL((k_line_reg<0, C>), dim3((unsigned)blocks), dim3(p.threads), lds, s, in, out, p, tw,
-                           sign, fg, ci);
-    else
-        hipLaunchKernelGGL((k_line_reg<1, C>), dim3((unsigned)blocks), dim3(p.threads), lds, s, in, out, p, tw,
-                           sign, fg, ci);
+    const dim3 grid((unsigned)blocks), block(p.threads);
+    const bool r12 = reg_rmax(p) == 12;
+    if (mode == 0) {
+        if (r12)
+            hipLaunchKernelGGL((k_line_reg<0, C, 12>), grid, block, lds, s, in, out, p, tw, sign, fg, ci);
+        else
+            hipLaunchKernelGGL((k_line_reg<0, C, 16>), grid, block, lds, s, in, out, p, tw, sign, fg, ci);
+    } else {
+        if (r12)
+            hipLaunchKernelGGL((k_line_reg<1, C, 12>), grid, block, lds, s, in, out, p, tw, sign, fg, ci);
+        else
+            hipLaunchKernelGGL((k_line_reg<1, C, 16>), grid, block, lds, s, in, out, p, tw, sign, fg, ci);
+    }
     HIP_CHECK(hipGetLastError());
 }
 

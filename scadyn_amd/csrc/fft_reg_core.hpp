@@ -344,49 +344,63 @@ HD Ctx make_ctx(const LinePass &p, long long block, int thread) {
 // of sign s): phase 0 = stage 1, phase 1 = stage 2.  MODE 1 (forward, x Fg,
 // inverse): phase 0 = forward stage 1, phase 1 = conv_compute (registers v),
 // phase 2 = conv_store, phase 3 = inverse stage 2.
-template <int MODE, typename C>
+template <int MODE, typename C, int RMAX = 16>
 HD void line_reg_phase(int phase, const C *__restrict__ in, C *__restrict__ out, C *lds, const C *tw,
                        const C *__restrict__ fg, const ConvInfo &ci, const LinePass &p, const Ctx &x, int sign,
                        C *v) {
+    // RMAX: largest radix compiled into this instance (fewer VGPRs for small radices)
     const int R1 = p.rad[0], R2 = p.rad[1];
     const int s1 = (MODE == 0) ? sign : -1;
     if (phase == 0) {
-#define FIRST(R)                                      \
-    case R:                                           \
-        if (s1 < 0)                                   \
-            stage_first<R, -1, C>(in, lds, p, x);     \
-        else                                          \
-            stage_first<R, +1, C>(in, lds, p, x);     \
+#define FIRST(R)                                          \
+    case R:                                               \
+        if constexpr (R <= RMAX) {                        \
+            if (s1 < 0)                                   \
+                stage_first<R, -1, C>(in, lds, p, x);     \
+            else                                          \
+                stage_first<R, +1, C>(in, lds, p, x);     \
+        }                                                 \
         break;
         switch (R1) { REG_RADICES(FIRST) }
 #undef FIRST
     } else if (MODE == 0) {
-#define LAST(R)                                             \
-    case R:                                                 \
-        if (s1 < 0)                                         \
-            stage_last<R, -1, C>(out, lds, tw, p, x);       \
-        else                                                \
-            stage_last<R, +1, C>(out, lds, tw, p, x);       \
+#define LAST(R)                                                 \
+    case R:                                                     \
+        if constexpr (R <= RMAX) {                              \
+            if (s1 < 0)                                         \
+                stage_last<R, -1, C>(out, lds, tw, p, x);       \
+            else                                                \
+                stage_last<R, +1, C>(out, lds, tw, p, x);       \
+        }                                                       \
         break;
         switch (R2) { REG_RADICES(LAST) }
 #undef LAST
     } else if (phase == 1) {
-#define MID(R) \
-    case R: conv_compute<R, C>(v, lds, tw, fg, ci, p, x); break;
+#define MID(R)                                                                     \
+    case R:                                                                        \
+        if constexpr (R <= RMAX) conv_compute<R, C>(v, lds, tw, fg, ci, p, x);     \
+        break;
         switch (R2) { REG_RADICES(MID) }
 #undef MID
     } else if (phase == 2) {
-#define STO(R) \
-    case R: conv_store<R, C>(v, lds, p, x); break;
+#define STO(R)                                                   \
+    case R:                                                      \
+        if constexpr (R <= RMAX) conv_store<R, C>(v, lds, p, x); \
+        break;
         switch (R2) { REG_RADICES(STO) }
 #undef STO
     } else {
-#define ILAST(R) \
-    case R: stage_last<R, +1, C>(out, lds, tw, p, x); break;
+#define ILAST(R)                                                            \
+    case R:                                                                 \
+        if constexpr (R <= RMAX) stage_last<R, +1, C>(out, lds, tw, p, x);  \
+        break;
         switch (R1) { REG_RADICES(ILAST) }
 #undef ILAST
     }
 }
+
+// smallest compiled radix bound covering a pass
+HD int reg_rmax(const LinePass &p) { return (p.rad[0] <= 12 && p.rad[1] <= 12) ? 12 : 16; }
 
 }  // namespace regfft
 }  // namespace scadyn

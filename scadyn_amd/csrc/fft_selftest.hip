@@ -42,11 +42,17 @@ static void emulate(const LinePass &p, const C *in, C *out, const C *twg, int si
     std::vector<C> lds((size_t)p.L * p.ti), tw(twg, twg + p.L);
     std::vector<std::array<C, 16>> regs(p.threads);
     const int nph = MODE == 0 ? 2 : 4;
+    const bool r12 = reg_rmax(p) == 12;  // the instance the launcher picks
     for (long long b = 0; b < blocks; ++b)
         for (int ph = 0; ph < nph; ++ph)
-            for (int t = 0; t < p.threads; ++t)
-                line_reg_phase<MODE, C>(ph, in, out, lds.data(), tw.data(), fg, ci, p, make_ctx(p, b, t), sign,
-                                        regs[t].data());
+            for (int t = 0; t < p.threads; ++t) {
+                if (r12)
+                    line_reg_phase<MODE, C, 12>(ph, in, out, lds.data(), tw.data(), fg, ci, p, make_ctx(p, b, t),
+                                                sign, regs[t].data());
+                else
+                    line_reg_phase<MODE, C, 16>(ph, in, out, lds.data(), tw.data(), fg, ci, p, make_ctx(p, b, t),
+                                                sign, regs[t].data());
+            }
 }
 
 template <typename C>
