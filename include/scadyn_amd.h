@@ -71,7 +71,10 @@ typedef struct scadyn_opts {
     int32_t device;      /* HIP device ordinal                                          */
     int32_t fft_pad;     /* 0: smallest 2,3,5,7-smooth length >= 2N-1 per axis;
                             1: the reference's 2N                                       */
-    double refine_tol;   /* mixed precision: outer FP64 relative residual target        */
+    double refine_tol;   /* mixed precision: a system is done when its true (FP64)
+                            residual at a restart is <= refine_tol ||b||; the Arnoldi
+                            steps use the FP32 matvec and tol is then the residual
+                            reduction that ends a restart cycle early               */
     int32_t ortho;       /* Arnoldi orthogonalisation: 0 modified Gram-Schmidt (the
                             reference's, src/gmres_module.f90:241-244), one pass per
                             basis vector; 1 classical Gram-Schmidt with DGKS
@@ -117,7 +120,7 @@ int scadyn_op_create(const scadyn_mesh *mesh, const scadyn_opts *opts, double k,
 void scadyn_op_destroy(scadyn_operator *op);
 
 /* Sizes: info[0..15] = N_tet, Nx, Ny, Nz, N_cubes, N_tet_cube, n_blocks,
- * Lx, Ly, Lz, batch, device bytes, refinement rounds of the last mixed-precision
+ * Lx, Ly, Lz, batch, device bytes, most restart cycles of a system in the last
  * solve, 1 if the near-zone rows are in cube order (per-cube gather), 0... ;
  * delta, box_delta and k in dinfo[0..2]. */
 int scadyn_op_info(const scadyn_operator *op, int64_t info[16], double dinfo[4]);
@@ -139,9 +142,9 @@ int scadyn_op_matvec_prec(scadyn_operator *op, int32_t precision, int32_t nrhs, 
 
 /* Solve A x = rhs for nrhs right-hand sides [nrhs][3N] with restarted GMRES
  * (src/gmres_module.f90:180-325).  iters[nrhs] (optional) receives the
- * reference's iteration count `ite` of each system (summed over the inner
- * solves in mixed precision, which refines with FP64 residuals until
- * ||b - A x|| <= refine_tol ||b||). */
+ * reference's iteration count `ite` of each system.  In mixed precision the
+ * Arnoldi steps use the FP32 matvec and every restart the FP64 true residual;
+ * a system is done when ||b - A x|| <= refine_tol ||b|| at a restart. */
 int scadyn_op_solve(scadyn_operator *op, int32_t nrhs, const double *rhs, double *x,
                     int32_t *iters);
 

@@ -119,21 +119,35 @@ struct Profiler {
 };
 
 // ---------------------------------------------------------------- workspace
+// Grids, FFT plans and input offsets of one batched matvec of B slots.
+struct MatvecSpace {
+    int B = 0;
+    DevMem mem;
+    void *grid0 = nullptr, *grid1 = nullptr;  // complex128 or complex64 grids (one matvec at a time)
+    long long *in_off = nullptr;              // slot b's input at xin + in_off[b] + row*B + b
+    int *active = nullptr;
+    LinePass pass[2][5];  // [fp64, fp32][x fwd, y fwd, z conv, y inv, x inv]
+};
+
 struct Workspace {
     int B = 0, m = 0;
     long long n = 0;
     DevMem mem;
-    void *grid0 = nullptr, *grid1 = nullptr;  // complex128 or complex64 grids (one matvec at a time)
+    MatvecSpace mv;  // the batch's matvec
+    // mixed precision: FP64 matvec of the restarting slots only, packed into
+    // a narrow batch of at most rs_cap slots (x in rx, A x in ry)
+    std::unique_ptr<MatvecSpace> rs;
+    int rs_cap = 0;
+    cd *rx = nullptr, *ry = nullptr;
+    int *rslot = nullptr;
     cd *V = nullptr, *Rhs = nullptr, *W = nullptr;
     SlotState *st = nullptr;
     cd *H = nullptr, *cs = nullptr, *sn = nullptr, *g = nullptr, *y = nullptr;
     double *nrm2 = nullptr, *nav2 = nullptr, *pnrm = nullptr, *pnav = nullptr;
     cd *pdot = nullptr;
-    long long *in_off = nullptr;
-    int *col = nullptr, *active = nullptr, *ret = nullptr, *flag = nullptr;
+    int *col = nullptr, *ret = nullptr, *flag = nullptr;
     cd *cc = nullptr, *hprev = nullptr;
     int nwg = 0;
-    LinePass pass[2][5];  // [fp64, fp32][x fwd, y fwd, z conv, y inv, x inv]
 };
 
 }  // namespace scadyn
@@ -366,6 +380,28 @@ static size_t slot_bytes(const scadyn_operator *op, int m) {
     return sizeof(cd) * ((size_t)(m + 4) * n + 2 * grid);
 }
 
+static void make_matvec_space(scadyn_operator *op, int B, MatvecSpace &ms) {
+    ms.B = B;
+    const int Lx = op->L[0], Ly = op->L[1], Lz = op->L[2];
+    const int Nx = op->g.Nx, Ny = op->g.Ny, Nz = op->g.Nz;
+    const long long nb = 4LL * B;
+    const size_t gsz = (size_t)Lx * Ly * Nz * nb;
+    ms.grid0 = ms.mem.alloc<cd>(gsz);
+    ms.grid1 = ms.mem.alloc<cd>(gsz);
+    ms.in_off = ms.mem.alloc<long long>(B);
+    ms.active = ms.mem.alloc<int>(B);
+    const size_t budget = 64 * 1024;
+    const bool reg = op->opts.fft_kernel == 0;
+    for (int pr = 0; pr < 2; ++pr) {
+        const size_t el = pr == 0 ? sizeof(cd) : sizeof(cf);
+        make_line_pass(ms.pass[pr][0], Lx, Nx, Lx, nb, (long long)Ny * Nz, budget, el, reg);
+        make_line_pass(ms.pass[pr][1], Ly, Ny, Ly, (long long)Lx * nb, Nz, budget, el, reg);
+        make_line_pass(ms.pass[pr][2], Lz, Nz, Nz, (long long)Ly * Lx * nb, 1, budget, el, reg);
+        make_line_pass(ms.pass[pr][3], Ly, Ly, Ny, (long long)Lx * nb, Nz, budget, el, reg);
+        make_line_pass(ms.pass[pr][4], Lx, Lx, Nx, nb, (long long)Ny * Nz, budget, el, reg);
+    }
+}
+
 static Workspace &workspace(scadyn_operator *op, int B) {
     const int m = op->opts.maxit;
     if (op->ws && op->ws->B == B && op->ws->m == m) return *op->ws;
@@ -375,12 +411,7 @@ static Workspace &workspace(scadyn_operator *op, int B) {
     w.m = m;
     w.n = 3LL * op->g.N;
     const size_t nB = (size_t)w.n * B;
-    const int Lx = op->L[0], Ly = op->L[1], Lz = op->L[2];
-    const int Nx = op->g.Nx, Ny = op->g.Ny, Nz = op->g.Nz;
-    const long long nb = 4LL * B;
-    const size_t gsz = (size_t)Lx * Ly * Nz * nb;
-    w.grid0 = w.mem.alloc<cd>(gsz);
-    w.grid1 = w.mem.alloc<cd>(gsz);
+    make_matvec_space(op, B, w.mv);
     w.V = w.mem.alloc<cd>((size_t)(m + 2) * nB);
     w.Rhs = w.mem.alloc<cd>(nB);
     w.W = w.mem.alloc<cd>(nB);
@@ -405,21 +436,24 @@ static Workspace &workspace(scadyn_operator *op, int B) {
     w.flag = w.mem.alloc<int>(B);
     w.cc = w.mem.alloc<cd>((size_t)B * std::max(m + 1, bmgs_partials() + 2));
     w.hprev = w.mem.alloc<cd>((size_t)B * (m + 1));
-    w.in_off = w.mem.alloc<long long>(B);
     w.col = w.mem.alloc<int>(B);
     w.ret = w.mem.alloc<int>(B);
-    w.active = w.mem.alloc<int>(B);
-    const size_t budget = 64 * 1024;
-    const bool reg = op->opts.fft_kernel == 0;
-    for (int pr = 0; pr < 2; ++pr) {
-        const size_t el = pr == 0 ? sizeof(cd) : sizeof(cf);
-        make_line_pass(w.pass[pr][0], Lx, Nx, Lx, nb, (long long)Ny * Nz, budget, el, reg);
-        make_line_pass(w.pass[pr][1], Ly, Ny, Ly, (long long)Lx * nb, Nz, budget, el, reg);
-        make_line_pass(w.pass[pr][2], Lz, Nz, Nz, (long long)Ly * Lx * nb, 1, budget, el, reg);
-        make_line_pass(w.pass[pr][3], Ly, Ly, Ny, (long long)Lx * nb, Nz, budget, el, reg);
-        make_line_pass(w.pass[pr][4], Lx, Lx, Nx, nb, (long long)Ny * Nz, budget, el, reg);
-    }
     return w;
+}
+
+static void h2d_sync(void *dst, const void *src, size_t bytes, hipStream_t s);
+
+// the narrow FP64 restart batch of a mixed-precision workspace (allocated once)
+static void restart_space(scadyn_operator *op, Workspace &w) {
+    if (w.rs) return;
+    w.rs_cap = std::min(w.B, 8);
+    w.rs.reset(new MatvecSpace());
+    make_matvec_space(op, w.rs_cap, *w.rs);
+    w.rx = w.mem.alloc<cd>((size_t)w.n * w.rs_cap);
+    w.ry = w.mem.alloc<cd>((size_t)w.n * w.rs_cap);
+    w.rslot = w.mem.alloc<int>(w.rs_cap);
+    std::vector<long long> off(w.rs_cap, 0);
+    h2d_sync(w.rs->in_off, off.data(), sizeof(long long) * w.rs_cap, op->s);
 }
 
 struct ProfScope {
@@ -454,7 +488,7 @@ struct OpData<cf> {
 // y = A x for the batch: x element (row, b) at xin[in_off[b] + row*B + b];
 // grids, S* and FFT(G) in precision C, x and y complex128
 template <typename C>
-static void run_matvec(scadyn_operator *op, Workspace &w, const cd *xin, const int *active, cd *y) {
+static void run_matvec(scadyn_operator *op, MatvecSpace &w, const cd *xin, const int *active, cd *y) {
     const HostGeometry &g = op->g;
     const int pr = OpData<C>::pr;
     C *g0 = (C *)w.grid0, *g1 = (C *)w.grid1;
@@ -582,11 +616,15 @@ static int auto_batch(scadyn_operator *op, int ncols) {
 }
 
 struct SolveSpec {
-    int prec = SCADYN_PREC_FP64;  // matvec precision
+    int prec = SCADYN_PREC_FP64;  // matvec precision of the Arnoldi steps
     bool x0zero = false;          // x0 = 0 (correction equations) instead of x0 = rhs
     double tol = 1e-5;
     int restart = 4;
     int ortho = 0;                // 0 MGS (reference), 1 CGS + DGKS, 2 block MGS
+    // mixed precision: every restart takes the true residual with the FP64
+    // matvec and a system is done only when it is <= tol_true ||b||
+    bool verify64 = false;
+    double tol_true = 0.0;
 };
 
 // Pinned host buffer (page-locked, so async copies never read a reused buffer).
@@ -624,9 +662,9 @@ struct ColumnSolver {
     KrylovDims d;
     int B = 0;
     Pinned<SlotState> hs;
-    Pinned<int> h_load, h_ret;
+    Pinned<int> h_load, h_ret, h_rs;
     std::vector<int> slot_col;
-    int next = 0, nfail = 0, singular = 0;
+    int next = 0, nfail = 0, singular = 0, max_cycles = 0;
     long long steps = 0, max_steps = 0;
     hipEvent_t ev = nullptr;
     bool pending = false;
@@ -653,12 +691,16 @@ struct ColumnSolver {
         hs.alloc(B);
         h_load.alloc(B);
         h_ret.alloc(B);
+        if (spec.verify64) {
+            restart_space(op, *w);
+            h_rs.alloc(((B + w->rs_cap - 1) / w->rs_cap) * w->rs_cap);
+        }
         std::memset(hs.p, 0, sizeof(SlotState) * B);
         slot_col.assign(B, -1);
         const size_t nB = (size_t)w->n * B;
         HIP_CHECK(hipMemsetAsync(w->st, 0, sizeof(SlotState) * B, op->s));
         HIP_CHECK(hipMemsetAsync(w->V, 0, sizeof(cd) * (size_t)(w->m + 2) * nB, op->s));
-        // every system finishes within restart*(m+1) matvecs; bound the loop
+        // every system finishes within restart cycles of at most m+1 matvecs; bound the loop
         max_steps = ((long long)spec.restart * (w->m + 1) + 4) * (((long long)cols.size() + B - 1) / B + 2) + 16;
         HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     }
@@ -698,11 +740,29 @@ struct ColumnSolver {
             if (hs.p[b].phase == PH_ARNOLDI) K = std::max(K, hs.p[b].i + 1);
         }
         if (K >= 0) {
-            launch_set_inputs(d, w->st, w->in_off, w->active, s);
-            if (spec.prec == SCADYN_PREC_MIXED)
-                run_matvec<cf>(op, *w, w->V, w->active, w->W);
-            else
-                run_matvec<cd>(op, *w, w->V, w->active, w->W);
+            // restarting slots (verify64): true residual from the FP64 matvec of x,
+            // packed into the narrow restart batch; the rest take the batch matvec
+            int nrst = 0, narn = 0;
+            for (int b = 0; b < B; ++b) {
+                if (slot_col[b] < 0) continue;
+                if (hs.p[b].phase == PH_RESTART && spec.verify64) h_rs.p[nrst++] = b;
+                if (hs.p[b].phase == PH_ARNOLDI || (hs.p[b].phase == PH_RESTART && !spec.verify64)) ++narn;
+            }
+            launch_set_inputs(d, w->st, w->mv.in_off, w->mv.active, spec.verify64 ? 1 : 0, s);
+            if (narn > 0) {
+                if (spec.prec == SCADYN_PREC_MIXED)
+                    run_matvec<cf>(op, w->mv, w->V, w->mv.active, w->W);
+                else
+                    run_matvec<cd>(op, w->mv, w->V, w->mv.active, w->W);
+            }
+            for (int r0 = 0; r0 < nrst; r0 += w->rs_cap) {
+                const int nr = std::min(w->rs_cap, nrst - r0);
+                for (int j = nr; j < w->rs_cap; ++j) h_rs.p[r0 + j] = -1;
+                HIP_CHECK(hipMemcpyAsync(w->rslot, h_rs.p + r0, sizeof(int) * w->rs_cap, hipMemcpyHostToDevice, s));
+                launch_pack_slots(d, w->V + (size_t)(w->m + 1) * w->n * B, w->rslot, w->rs_cap, w->rx, s);
+                run_matvec<cd>(op, *w->rs, w->rx, nullptr, w->ry);
+                launch_unpack_slots(d, w->ry, w->rslot, w->rs_cap, w->W, s);
+            }
             ProfScope ps(op, SCADYN_KCLASS_KRYLOV, 0.0);
             if (spec.ortho == 1) {
                 launch_cgs(d, w->W, w->V, w->Rhs, w->st, w->H, w->pdot, w->pnrm, w->pnav, w->nrm2, w->nav2, w->flag,
@@ -717,7 +777,7 @@ struct ColumnSolver {
                 }
             }
             launch_slot_update(d, w->st, w->H, w->cs, w->sn, w->g, w->y, w->nrm2, w->nav2, spec.tol, spec.restart,
-                               s);
+                               spec.verify64 ? 1 : 0, spec.tol_true, s);
             launch_vec_update(d, w->st, w->W, w->V, w->y, s);
         }
         HIP_CHECK(hipMemcpyAsync(hs.p, w->st, sizeof(SlotState) * B, hipMemcpyDeviceToHost, s));
@@ -736,6 +796,7 @@ struct ColumnSolver {
             if (slot_col[b] >= 0 && hs.p[b].phase == PH_DONE) {
                 h_ret.p[b] = slot_col[b];
                 if (iters) iters[slot_col[b]] += hs.p[b].ite;
+                max_cycles = std::max(max_cycles, hs.p[b].cycle + 1);
                 if (hs.p[b].conv == 0) nfail++;
                 if (hs.p[b].conv < 0) singular++;
                 slot_col[b] = -1;
@@ -751,163 +812,60 @@ struct ColumnSolver {
     }
 };
 
-// Solve A x = src(:, c) for every column c of `cols` (src, dst: n x ld, internal
-// rows); continuous batching: a slot is refilled as soon as its system finishes.
-static void solve_columns(scadyn_operator *op, const cd *src, long long ld, const std::vector<int> &cols, cd *dst,
-                          int32_t *iters, int *nconv_fail, const SolveSpec &spec) {
-    if (cols.empty()) return;
+// Solver settings of the operator's precision mode.  FP64: the reference's
+// GMRES.  Mixed: the Arnoldi steps take the FP32 matvec, every restart takes
+// the true residual with the FP64 matvec, and a system is done when that
+// residual is <= refine_tol ||b|| (an Arnoldi estimate below tol only ends the
+// cycle early): iterative refinement with one restart cycle per correction.
+static SolveSpec solve_spec(const scadyn_operator *op) {
+    SolveSpec spec;
+    spec.tol = op->opts.tol;
+    spec.restart = op->opts.restart;
+    spec.ortho = op->opts.ortho;
+    if (op->opts.precision == SCADYN_PREC_MIXED) {
+        spec.prec = SCADYN_PREC_MIXED;
+        spec.verify64 = true;
+        spec.tol_true = op->opts.refine_tol;
+    }
+    return spec;
+}
+
+// Solve A X = Bm for all ld columns.
+static void solve_all(scadyn_operator *op, const cd *Bm, long long n, int ld, cd *X, int32_t *iters, int *nfail,
+                      int32_t *outer_rounds) {
+    (void)n;
+    std::vector<int> all(ld);
+    for (int c = 0; c < ld; ++c) all[c] = c;
+    if (iters)
+        for (int c = 0; c < ld; ++c) iters[c] = 0;
+    const SolveSpec spec = solve_spec(op);
     ColumnSolver cs;
-    cs.init(op, src, ld, cols, dst, iters, spec);
+    cs.init(op, Bm, ld, all, X, iters, spec);
     while (!cs.done()) {
         cs.launch();
         cs.finish();
     }
     HIP_CHECK(hipStreamSynchronize(op->s));
     if (cs.singular) fail(SCADYN_ERR_SINGULAR, "Matrix is numerically singular! (GMRES least squares)");
-    if (nconv_fail) *nconv_fail += cs.nfail;
-}
-
-// ---- column utilities for the refinement loop (n x ld row-major, internal rows)
-__global__ void k_col_norm2(const cd *__restrict__ A, long long n, int ld, double *__restrict__ part) {
-    // part[blockIdx.x][col]; threads stride over rows for one column each
-    for (int c = threadIdx.x; c < ld; c += blockDim.x) {
-        double acc = 0.0;
-        for (long long r = blockIdx.x; r < n; r += gridDim.x) acc += cabs2(A[(size_t)r * ld + c]);
-        part[(size_t)blockIdx.x * ld + c] = acc;
-    }
-}
-__global__ void k_add_cols(cd *__restrict__ X, const cd *__restrict__ D, long long n, int ld,
-                           const int *__restrict__ mask) {
-    const long long tot = n * ld;
-    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
-         e += (long long)gridDim.x * blockDim.x)
-        if (mask[e % ld]) X[e] = cadd(X[e], D[e]);
-}
-// gather columns cols[b] of X into the batch layout [n][B]
-__global__ void k_gather_cols(const cd *__restrict__ X, long long n, int ld, const int *__restrict__ cols, int B,
-                              cd *__restrict__ out) {
-    const long long tot = n * B;
-    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
-         e += (long long)gridDim.x * blockDim.x) {
-        const int b = (int)(e % B);
-        const long long r = e / B;
-        out[e] = cols[b] >= 0 ? X[(size_t)r * ld + cols[b]] : cmk(0, 0);
-    }
-}
-// R(:, cols[b]) = Bm(:, cols[b]) - AX(:, b)
-__global__ void k_resid_cols(const cd *__restrict__ Bm, const cd *__restrict__ AX, long long n, int ld,
-                             const int *__restrict__ cols, int B, cd *__restrict__ R) {
-    const long long tot = n * B;
-    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
-         e += (long long)gridDim.x * blockDim.x) {
-        const int b = (int)(e % B);
-        if (cols[b] < 0) continue;
-        const long long r = e / B;
-        const size_t o = (size_t)r * ld + cols[b];
-        R[o] = csub(Bm[o], AX[e]);
-    }
-}
-
-static std::vector<double> col_norms(scadyn_operator *op, const cd *A, long long n, int ld) {
-    const int nb = 256;
-    DevMem tmp;
-    double *part = tmp.alloc<double>((size_t)nb * ld);
-    hipLaunchKernelGGL(k_col_norm2, dim3(nb), dim3(256), 0, op->s, A, n, ld, part);
-    std::vector<double> h((size_t)nb * ld), out(ld, 0.0);
-    HIP_CHECK(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, op->s));
-    HIP_CHECK(hipStreamSynchronize(op->s));
-    for (int b = 0; b < nb; ++b)
-        for (int c = 0; c < ld; ++c) out[c] += h[(size_t)b * ld + c];
-    for (auto &v : out) v = std::sqrt(v);
-    return out;
-}
-
-// R(:, c) = Bm(:, c) - A X(:, c) with the FP64 operator, for c in cols
-static void fp64_residual(scadyn_operator *op, const cd *Bm, const cd *X, long long n, int ld,
-                          const std::vector<int> &cols, cd *R) {
-    const int B = auto_batch(op, (int)cols.size());
-    Workspace &w = workspace(op, B);
-    std::vector<long long> off(B, 0);
-    h2d_sync(w.in_off, off.data(), sizeof(long long) * B, op->s);
-    for (size_t c0 = 0; c0 < cols.size(); c0 += B) {
-        std::vector<int> cb(B, -1);
-        for (int b = 0; b < B && c0 + b < cols.size(); ++b) cb[b] = cols[c0 + b];
-        h2d_sync(w.col, cb.data(), sizeof(int) * B, op->s);
-        hipLaunchKernelGGL(k_gather_cols, dim3(nblk(n * B)), dim3(256), 0, op->s, X, n, ld, w.col, B, w.Rhs);
-        run_matvec<cd>(op, w, w.Rhs, nullptr, w.W);
-        hipLaunchKernelGGL(k_resid_cols, dim3(nblk(n * B)), dim3(256), 0, op->s, Bm, w.W, n, ld, w.col, B, R);
-    }
-    HIP_CHECK(hipStreamSynchronize(op->s));
-}
-
-// Solve A X = Bm for all ld columns.  FP64: the reference's GMRES.  Mixed:
-// iterative refinement, inner GMRES on the FP32 matvec from x0 = 0 to
-// opts.tol, outer FP64 residual until ||b - A x|| <= refine_tol ||b||.
-static void solve_all(scadyn_operator *op, const cd *Bm, long long n, int ld, cd *X, int32_t *iters, int *nfail,
-                      int32_t *outer_rounds) {
-    std::vector<int> all(ld);
-    for (int c = 0; c < ld; ++c) all[c] = c;
-    if (iters)
-        for (int c = 0; c < ld; ++c) iters[c] = 0;
-    SolveSpec spec;
-    spec.tol = op->opts.tol;
-    spec.restart = op->opts.restart;
-    spec.ortho = op->opts.ortho;
-    if (op->opts.precision != SCADYN_PREC_MIXED) {
-        solve_columns(op, Bm, ld, all, X, iters, nfail, spec);
-        if (outer_rounds) *outer_rounds = 1;
-        return;
-    }
-    spec.prec = SCADYN_PREC_MIXED;
-    spec.x0zero = true;
-    DevMem tmp;
-    cd *R = tmp.alloc<cd>((size_t)n * ld);
-    cd *D = tmp.alloc<cd>((size_t)n * ld);
-    int *mask = tmp.alloc<int>(ld);
-    HIP_CHECK(hipMemsetAsync(X, 0, sizeof(cd) * (size_t)n * ld, op->s));
-    HIP_CHECK(hipMemcpyAsync(R, Bm, sizeof(cd) * (size_t)n * ld, hipMemcpyDeviceToDevice, op->s));
-    std::vector<double> bn = col_norms(op, Bm, n, ld);
-    std::vector<int> active = all;
-    int rounds = 0;
-    for (int outer = 0; outer < 30 && !active.empty(); ++outer) {
-        ++rounds;
-        int nf = 0;
-        solve_columns(op, R, ld, active, D, iters, &nf, spec);
-        std::vector<int> m(ld, 0);
-        for (int c : active) m[c] = 1;
-        h2d_sync(mask, m.data(), sizeof(int) * ld, op->s);
-        hipLaunchKernelGGL(k_add_cols, dim3(nblk(n * ld)), dim3(256), 0, op->s, X, D, n, ld, mask);
-        fp64_residual(op, Bm, X, n, ld, active, R);
-        std::vector<double> rn = col_norms(op, R, n, ld);
-        std::vector<int> still;
-        for (int c : active)
-            if (!(rn[c] <= op->opts.refine_tol * bn[c])) still.push_back(c);
-        active.swap(still);
-    }
-    if (nfail) *nfail += (int)active.size();
-    if (outer_rounds) *outer_rounds = rounds;
+    if (nfail) *nfail += cs.nfail;
+    if (outer_rounds) *outer_rounds = cs.max_cycles;
 }
 
 // One T-matrix solve (compute_T_matrix, src/T_matrix.f90:240-277) as a
 // resumable job: launch()/finish() advance its column solver by one step, so a
 // host loop can interleave the steps of several jobs on different streams.
-// FP64: one batched GMRES over the 2nm columns.  Mixed: rounds of FP32-matvec
-// GMRES on the residuals with FP64 residual updates between rounds.
+// All 2nm columns run as one continuously refilled batch (solve_spec gives
+// the precision mode).
 struct TmatJob {
     scadyn_operator *op = nullptr;
     int nmax = 0, nm = 0, ncol = 0;
     long long n = 0;
-    bool mixed = false;
     DevMem tmp;
-    cd *mat = nullptr, *X = nullptr, *R = nullptr, *D = nullptr;
-    int *mask = nullptr;
-    std::vector<double> bn;
-    std::vector<int> active;
+    cd *mat = nullptr, *X = nullptr;
     std::vector<int32_t> iters;
     int rounds = 0, nfail = 0;
     bool finished = false;
     std::unique_ptr<ColumnSolver> cs;
-    SolveSpec spec;
 
     void start(scadyn_operator *o, int nmax_) {
         op = o;
@@ -916,31 +874,15 @@ struct TmatJob {
         nm = (nmax + 1) * (nmax + 1) - 1;
         ncol = 2 * nm;
         n = 3LL * op->g.N;
-        mixed = op->opts.precision == SCADYN_PREC_MIXED;
         hipStream_t s = op->s;
         mat = tmp.alloc<cd>((size_t)n * ncol);
         launch_vswf(op->d_tcoord, op->d_vol, op->g.N, nmax, op->k, mat, ncol, s);
         X = tmp.alloc<cd>((size_t)n * ncol);
         iters.assign(ncol, 0);
-        active.resize(ncol);
-        for (int c = 0; c < ncol; ++c) active[c] = c;
-        spec.tol = op->opts.tol;
-        spec.restart = op->opts.restart;
-        spec.ortho = op->opts.ortho;
+        std::vector<int> all(ncol);
+        for (int c = 0; c < ncol; ++c) all[c] = c;
         cs.reset(new ColumnSolver());
-        if (!mixed) {
-            cs->init(op, mat, ncol, active, X, iters.data(), spec);
-            return;
-        }
-        spec.prec = SCADYN_PREC_MIXED;
-        spec.x0zero = true;
-        R = tmp.alloc<cd>((size_t)n * ncol);
-        D = tmp.alloc<cd>((size_t)n * ncol);
-        mask = tmp.alloc<int>(ncol);
-        HIP_CHECK(hipMemsetAsync(X, 0, sizeof(cd) * (size_t)n * ncol, s));
-        HIP_CHECK(hipMemcpyAsync(R, mat, sizeof(cd) * (size_t)n * ncol, hipMemcpyDeviceToDevice, s));
-        bn = col_norms(op, mat, n, ncol);
-        cs->init(op, R, ncol, active, D, iters.data(), spec);
+        cs->init(op, mat, ncol, all, X, iters.data(), solve_spec(op));
     }
 
     void launch() {
@@ -953,29 +895,9 @@ struct TmatJob {
         if (!cs->done()) return;
         HIP_CHECK(hipStreamSynchronize(op->s));
         if (cs->singular) fail(SCADYN_ERR_SINGULAR, "Matrix is numerically singular! (GMRES least squares)");
-        ++rounds;
-        if (!mixed) {
-            nfail += cs->nfail;
-            finished = true;
-            return;
-        }
-        std::vector<int> m(ncol, 0);
-        for (int c : active) m[c] = 1;
-        h2d_sync(mask, m.data(), sizeof(int) * ncol, op->s);
-        hipLaunchKernelGGL(k_add_cols, dim3(nblk(n * ncol)), dim3(256), 0, op->s, X, D, n, ncol, mask);
-        fp64_residual(op, mat, X, n, ncol, active, R);
-        std::vector<double> rn = col_norms(op, R, n, ncol);
-        std::vector<int> still;
-        for (int c : active)
-            if (!(rn[c] <= op->opts.refine_tol * bn[c])) still.push_back(c);
-        active.swap(still);
-        if (active.empty() || rounds >= 30) {
-            nfail += (int)active.size();
-            finished = true;
-            return;
-        }
-        cs.reset(new ColumnSolver());
-        cs->init(op, R, ncol, active, D, iters.data(), spec);
+        nfail += cs->nfail;
+        rounds = cs->max_cycles;
+        finished = true;
     }
 
     // T = i k^3 mat^H X split into the four nm x nm column-major blocks
@@ -1352,11 +1274,11 @@ int scadyn_op_matvec_prec(scadyn_operator *op, int32_t precision, int32_t nrhs, 
     HIP_CHECK(hipMemcpyAsync(hx, x, sizeof(cd) * nv, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_pack, dim3(nblk(3LL * N * nrhs)), dim3(256), 0, s, hx, nrhs, N, op->d_perm, nrhs, xin);
     std::vector<long long> off(nrhs, 0);
-    h2d_sync(w.in_off, off.data(), sizeof(long long) * nrhs, s);
+    h2d_sync(w.mv.in_off, off.data(), sizeof(long long) * nrhs, s);
     if (precision == SCADYN_PREC_MIXED)
-        run_matvec<cf>(op, w, xin, nullptr, w.W);
+        run_matvec<cf>(op, w.mv, xin, nullptr, w.W);
     else
-        run_matvec<cd>(op, w, xin, nullptr, w.W);
+        run_matvec<cd>(op, w.mv, xin, nullptr, w.W);
     hipLaunchKernelGGL(k_unpack, dim3(nblk(3LL * N * nrhs)), dim3(256), 0, s, w.W, nrhs, N, op->d_perm, nrhs, hx);
     HIP_CHECK(hipMemcpyAsync(y, hx, sizeof(cd) * nv, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));

@@ -55,7 +55,7 @@ struct Item {
     int c, cx, cy, cz, t0, t1;
     int b;       // slot used for loads (B - 1 for idle lanes of a partial chunk)
     bool valid;  // lane holds a real slot
-    bool act;    // slot active (inactive slots get y = 0)
+    bool act;    // slot active (inactive slots are not written)
     size_t base; // first box node of the cube
 };
 
@@ -168,13 +168,13 @@ GC_HD void chunk_part(const GatherArgs<C> &g, const Item &it, int nt, int v0, in
 
 template <typename C, int GT>
 GC_HD void write_out(const GatherArgs<C> &g, const Item &it, int tp, int nt, C (*acc)[3]) {
-    if (!it.valid) return;
+    if (!it.valid || !it.act) return;  // inactive slots are not written
 #pragma unroll
     for (int u = 0; u < GT; ++u) {
         if (u >= nt) break;
         cd *yo = g.y + (size_t)(3 * (tp + u)) * g.B + it.b;
 #pragma unroll
-        for (int r = 0; r < 3; ++r) yo[(size_t)r * g.B] = it.act ? Prec<cd>::from(acc[u][r]) : cmk(0, 0);
+        for (int r = 0; r < 3; ++r) yo[(size_t)r * g.B] = Prec<cd>::from(acc[u][r]);
     }
 }
 

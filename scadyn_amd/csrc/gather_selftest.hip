@@ -8,7 +8,8 @@
 //       + sum_{near tets v} A(t, v) x_v
 // on a synthetic cube lattice: 0 to 11 tets per cube (empty cubes, cubes with
 // more tets than one staging pass holds), near-zone entries pointing at
-// shuffled blocks with and without the transpose flag, inactive slots, and
+// shuffled blocks with and without the transpose flag, inactive slots (left
+// unwritten), and
 // batch widths 64, 112 (partial second chunk) and 56.
 //
 // Exit status 0 when every case agrees to 1e-13 (complex128) / 1e-5 (complex64).
@@ -122,6 +123,7 @@ static void run_case(int B, std::mt19937_64 &rng) {
         const int c = tet_cube[t], cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
         for (int b = 0; b < B; ++b) {
             cd acc[3] = {cmk(0, 0), cmk(0, 0), cmk(0, 0)};
+            if (!active[b]) acc[0] = acc[1] = acc[2] = cmk(-7.0, -7.0);  // not written
             if (active[b]) {
                 for (int j = 0; j < 27; ++j) {
                     const int jx = j % 3, jy = (j / 3) % 3, jz = j / 9;

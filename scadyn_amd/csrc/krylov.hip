@@ -353,10 +353,15 @@ __global__ __launch_bounds__(RED_T) void k_cgs_norm(KrylovDims d, int pass, cons
 }
 
 // per-slot scalar work: Givens rotations, convergence test, least squares
+// verify: restart residuals are true (FP64) residuals; a slot is done only when
+// one is <= tol_true ||b||, and an Arnoldi estimate below tol times the cycle's
+// starting residual ends the cycle early (x update, then a restart that checks
+// the true residual)
 __global__ void k_slot_update(KrylovDims d, SlotState *__restrict__ st, cd *__restrict__ Hall,
                               cd *__restrict__ csall, cd *__restrict__ snall, cd *__restrict__ gall,
                               cd *__restrict__ yall, const double *__restrict__ nrm2,
-                              const double *__restrict__ nav2, double tol, int restart) {
+                              const double *__restrict__ nav2, double tol, int restart, int verify,
+                              double tol_true) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= d.B) return;
     SlotState s = st[b];
@@ -373,6 +378,9 @@ __global__ void k_slot_update(KrylovDims d, SlotState *__restrict__ st, cd *__re
         g[0] = cmk(beta, 0);
         s.beta = beta;
         if (beta == 0.0) {  // x is exact; the reference would divide by zero here
+            s.phase = PH_DONE;
+            s.conv = 1;
+        } else if (verify && beta <= tol_true * s.bnorm) {
             s.phase = PH_DONE;
             s.conv = 1;
         } else {
@@ -405,12 +413,19 @@ __global__ void k_slot_update(KrylovDims d, SlotState *__restrict__ st, cd *__re
             g[i] = csub(cmul(cs[i], t1), cmul(sn[i], t2));
             g[i + 1] = cadd(cmul(sn[i], t1), cmul(cconj(cs[i]), t2));
         }
-        s.err = sqrt(cabs2(g[i + 1])) / s.bnorm;
+        // the reference's |g(i+1)|/||b||; with verify, relative to the cycle's
+        // starting (true) residual, so that every cycle is one correction step
+        s.err = sqrt(cabs2(g[i + 1])) / (verify ? s.beta : s.bnorm);
         int len = 0;
         if (s.err < tol) {
             len = i + 1;
-            s.conv = 1;
-            s.phase = PH_DONE;
+            if (verify) {
+                s.cycle++;
+                s.phase = (s.cycle >= restart) ? PH_DONE : PH_RESTART;
+            } else {
+                s.conv = 1;
+                s.phase = PH_DONE;
+            }
         } else {
             s.ite++;
             if (i + 1 == m) {
@@ -544,7 +559,7 @@ __global__ void k_store_x(KrylovDims d, const int *__restrict__ col, const cd *_
 // matvec input offsets and activity from the slot phases: a restarting slot
 // multiplies x (block m+1 of V), an Arnoldi slot its current basis vector
 __global__ void k_set_inputs(KrylovDims d, const SlotState *__restrict__ st, long long *__restrict__ in_off,
-                             int *__restrict__ active) {
+                             int *__restrict__ active, int restart_elsewhere) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= d.B) return;
     const long long nB = d.n * d.B;
@@ -553,7 +568,7 @@ __global__ void k_set_inputs(KrylovDims d, const SlotState *__restrict__ st, lon
     int a = 0;
     if (s.phase == PH_RESTART) {
         off = (long long)(d.m + 1) * nB;
-        a = 1;
+        a = restart_elsewhere ? 0 : 1;
     } else if (s.phase == PH_ARNOLDI) {
         off = (long long)s.i * nB;
         a = 1;
@@ -634,6 +649,32 @@ __global__ void k_tgemm_reduce(const cd *__restrict__ part, int nz, int ncol, cd
     }
 }
 
+// restart batch: columns slot[j] (j < cap, -1 none) of X (n x B) into the
+// packed n x cap batch, and the packed results back into W (n x B)
+__global__ void k_pack_slots(KrylovDims d, const cd *__restrict__ X, const int *__restrict__ slot, int cap,
+                             cd *__restrict__ out) {
+    const long long tot = d.n * cap;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int j = (int)(e % cap);
+        const long long row = e / cap;
+        const int b = slot[j];
+        out[e] = b >= 0 ? X[(size_t)row * d.B + b] : cmk(0.0, 0.0);
+    }
+}
+__global__ void k_unpack_slots(KrylovDims d, const cd *__restrict__ in, const int *__restrict__ slot, int cap,
+                               cd *__restrict__ W) {
+    const long long tot = d.n * cap;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int j = (int)(e % cap);
+        const int b = slot[j];
+        if (b < 0) continue;
+        const long long row = e / cap;
+        W[(size_t)row * d.B + b] = in[e];
+    }
+}
+
 // ---- launchers -------------------------------------------------------------------------
 static dim3 rs_grid(const KrylovDims &d) {
     int SB = std::min(d.B, 256);
@@ -652,9 +693,10 @@ void launch_mgs_reduce(const KrylovDims &d, int k, const SlotState *st, cd *H, c
     HIP_CHECK(hipGetLastError());
 }
 void launch_slot_update(const KrylovDims &d, SlotState *st, cd *H, cd *cs, cd *sn, cd *g, cd *y,
-                        const double *nrm2, const double *nav2, double tol, int restart, hipStream_t s) {
+                        const double *nrm2, const double *nav2, double tol, int restart, int verify, double tol_true,
+                        hipStream_t s) {
     hipLaunchKernelGGL(k_slot_update, dim3((d.B + 63) / 64), dim3(64), 0, s, d, st, H, cs, sn, g, y, nrm2, nav2,
-                       tol, restart);
+                       tol, restart, verify, tol_true);
     HIP_CHECK(hipGetLastError());
 }
 void launch_vec_update(const KrylovDims &d, const SlotState *st, const cd *W, cd *V, const cd *y,
@@ -713,8 +755,22 @@ void launch_cgs(const KrylovDims &d, cd *W, const cd *V, const cd *Rhs, const Sl
     HIP_CHECK(hipGetLastError());
 }
 
-void launch_set_inputs(const KrylovDims &d, const SlotState *st, long long *in_off, int *active, hipStream_t s) {
-    hipLaunchKernelGGL(k_set_inputs, dim3((d.B + 63) / 64), dim3(64), 0, s, d, st, in_off, active);
+void launch_pack_slots(const KrylovDims &d, const cd *X, const int *slot, int cap, cd *out, hipStream_t s) {
+    const long long tot = d.n * cap;
+    const unsigned g = (unsigned)std::min<long long>((tot + 255) / 256, 1 << 16);
+    hipLaunchKernelGGL(k_pack_slots, dim3(g), dim3(256), 0, s, d, X, slot, cap, out);
+    HIP_CHECK(hipGetLastError());
+}
+void launch_unpack_slots(const KrylovDims &d, const cd *in, const int *slot, int cap, cd *W, hipStream_t s) {
+    const long long tot = d.n * cap;
+    const unsigned g = (unsigned)std::min<long long>((tot + 255) / 256, 1 << 16);
+    hipLaunchKernelGGL(k_unpack_slots, dim3(g), dim3(256), 0, s, d, in, slot, cap, W);
+    HIP_CHECK(hipGetLastError());
+}
+void launch_set_inputs(const KrylovDims &d, const SlotState *st, long long *in_off, int *active,
+                       int restart_elsewhere, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_inputs, dim3((d.B + 63) / 64), dim3(64), 0, s, d, st, in_off, active,
+                       restart_elsewhere);
     HIP_CHECK(hipGetLastError());
 }
 void launch_retire(const KrylovDims &d, const int *col, SlotState *st, hipStream_t s) {

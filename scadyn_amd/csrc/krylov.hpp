@@ -39,8 +39,12 @@ void launch_mgs_pass(const KrylovDims &d, int k, cd *W, const cd *V, const cd *R
                      const cd *H, cd *pdot, double *pnrm, double *pnav, hipStream_t s);
 void launch_mgs_reduce(const KrylovDims &d, int k, const SlotState *st, cd *H, const cd *pdot,
                        const double *pnrm, const double *pnav, double *nrm2, double *nav2, hipStream_t s);
+// verify = 1: restart residuals are true FP64 residuals and decide convergence
+// against tol_true (mixed precision); an Arnoldi estimate below tol times the
+// cycle's starting residual ends a cycle
 void launch_slot_update(const KrylovDims &d, SlotState *st, cd *H, cd *cs, cd *sn, cd *g, cd *y,
-                        const double *nrm2, const double *nav2, double tol, int restart, hipStream_t s);
+                        const double *nrm2, const double *nav2, double tol, int restart, int verify, double tol_true,
+                        hipStream_t s);
 void launch_vec_update(const KrylovDims &d, const SlotState *st, const cd *W, cd *V, const cd *y,
                        hipStream_t s);
 // load column col[b] (>= 0) of src (n x ld, row-major) into Rhs and X of slot b
@@ -64,8 +68,14 @@ void launch_bmgs(const KrylovDims &d, cd *W, const cd *V, const cd *Rhs, const S
                  double *pnrm, double *pnav, cd *red, double *nrm2, double *nav2, int kmax, hipStream_t s);
 int bmgs_partials();
 
-// matvec input offsets / activity derived on the device from the slot phases
-void launch_set_inputs(const KrylovDims &d, const SlotState *st, long long *in_off, int *active, hipStream_t s);
+// matvec input offsets / activity derived on the device from the slot phases;
+// restart_elsewhere = 1 leaves restarting slots inactive (their residual
+// matvec runs in the packed restart batch)
+void launch_set_inputs(const KrylovDims &d, const SlotState *st, long long *in_off, int *active,
+                       int restart_elsewhere, hipStream_t s);
+// columns slot[j] (j < cap, -1 none) of X (n x B) <-> a packed n x cap batch
+void launch_pack_slots(const KrylovDims &d, const cd *X, const int *slot, int cap, cd *out, hipStream_t s);
+void launch_unpack_slots(const KrylovDims &d, const cd *in, const int *slot, int cap, cd *W, hipStream_t s);
 // mark slots with col[b] >= 0 idle
 void launch_retire(const KrylovDims &d, const int *col, SlotState *st, hipStream_t s);
 

@@ -73,10 +73,7 @@ __global__ __launch_bounds__(256) void k_gather_near(
         const int b = (int)(e % B);
         const int t = (int)(e / B);
         cd *yo = y + (size_t)(3 * t) * B + b;
-        if (slot_active && !slot_active[b]) {
-            yo[0] = yo[(size_t)B] = yo[(size_t)2 * B] = cmk(0, 0);
-            continue;
-        }
+        if (slot_active && !slot_active[b]) continue;  // inactive slots are not written
         const int c = tet_cube[t];
         const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
         const size_t base = (size_t)(2 * cx) + (size_t)Nx * ((size_t)(2 * cy) + (size_t)Ny * (2 * cz));
@@ -204,10 +201,7 @@ __global__ __launch_bounds__(256) void k_gather_near_w64(
         const int b = valid ? braw : B - 1;
         if (!valid) continue;
         cd *yo = y + (size_t)(3 * t) * B + b;
-        if (slot_active && !slot_active[b]) {
-            yo[0] = yo[(size_t)B] = yo[(size_t)2 * B] = cmk(0, 0);
-            continue;
-        }
+        if (slot_active && !slot_active[b]) continue;  // inactive slots are not written
         const int c = tet_cube[t];
         const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
         const size_t base = (size_t)(2 * cx) + (size_t)Nx * ((size_t)(2 * cy) + (size_t)Ny * (2 * cz));
@@ -454,8 +448,9 @@ __global__ __launch_bounds__(256) void k_gather_cube_w64(
             for (int u = 0; u < GT; ++u) {
                 if (u >= nt || !valid) break;
                 cd *yo = y + (size_t)(3 * (tp + u)) * B + b;
+                if (act)
 #pragma unroll
-                for (int r = 0; r < 3; ++r) yo[(size_t)r * B] = act ? Prec<cd>::from(acc[u][r]) : cmk(0, 0);
+                    for (int r = 0; r < 3; ++r) yo[(size_t)r * B] = Prec<cd>::from(acc[u][r]);
             }
         }
     }

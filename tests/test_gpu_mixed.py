@@ -63,3 +63,27 @@ def test_mixed_solve_and_tmatrix(case):
     op64.close()
     opm.close()
     opc.close()
+
+
+def test_mixed_restart_batches(case):
+    """20 systems restarting together (more than one packed FP64 restart batch of
+    8) with short cycles (maxit 10): every system reaches the true-residual
+    target; solutions within 1e-8 of tight FP64 ones."""
+    from scadyn_amd import native
+    c, e, param, k = case
+    rng = np.random.default_rng(8)
+    b = rng.standard_normal((20, 3 * e.shape[0])) + 1j * rng.standard_normal((20, 3 * e.shape[0]))
+    say('fp64 tight, 20 systems')
+    op64 = native.Operator(c, e, param, k, tol=1e-11, restart=20)
+    x64, _ = op64.solve(b)
+    say('mixed, maxit 10')
+    opm = native.Operator(c, e, param, k, precision=native.PREC_MIXED, tol=1e-3, refine_tol=1e-10, maxit=10,
+                          restart=40)
+    xm, _ = opm.solve(b)
+    cycles = opm.info()['outer_rounds']
+    say(f'restart cycles {cycles}')
+    assert cycles >= 2
+    for r in range(20):
+        assert rel(xm[r], x64[r]) < 1e-8
+    op64.close()
+    opm.close()
