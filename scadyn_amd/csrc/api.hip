@@ -1306,6 +1306,7 @@ int scadyn_op_solve(scadyn_operator *op, int32_t nrhs, const double *rhs, double
     int nfail = 0;
     int32_t rounds = 0;
     solve_all(op, src, 3LL * N, nrhs, dst, iters, &nfail, &rounds);
+    op->last_outer_rounds = rounds;
     hipLaunchKernelGGL(k_unpack, dim3(nblk(3LL * N * nrhs)), dim3(256), 0, s, dst, nrhs, N, op->d_perm, nrhs, hx);
     HIP_CHECK(hipMemcpyAsync(x, hx, sizeof(cd) * nv, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
