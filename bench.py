@@ -5,6 +5,11 @@ Workload (the headline configuration): Gaussian random ellipsoid, semi-axes
 20 deg, real spherical harmonics l <= 10 (seed 0), cube-split to 198,390 tets,
 eps = 3.0 + 0.1i, size parameter ka = 8 (a = volume-equivalent radius),
 nmax = 12, i.e. 336 right-hand sides solved as one batched launch, FP64.
+The reference's GMRES settings apply (tol 1e-5, Krylov dimension 50, 4
+restart cycles).  The same solve in mixed precision (FP32 matvecs in the
+Arnoldi steps, FP64 true residual at every restart, same iteration budget and
+tolerance) follows by default; its columns/s, the ratio to FP64 and the
+relative difference of the two T-matrices are reported beside the FP64 value.
 
 A step is one full T-matrix solve (compute_T_matrix: VSWF right-hand sides,
 336 restarted-GMRES systems, projection GEMM) on the device-resident operator;
@@ -30,6 +35,7 @@ METRIC = 'T-matrix columns/s, mixed vs FP64 pFFT-GMRES at 1e-6 rel err; p99 late
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFLOPS = 78.6
 ORTHO_NAMES = {0: 'mgs', 1: 'cgs+dgks', 2: 'block-mgs'}
+DTYPES = {'fp64': 'f64', 'mixed': 'f32 matvec + f64 refinement'}
 
 from scadyn_amd.workloads import make_workload, request_stream  # noqa: E402
 
@@ -39,7 +45,7 @@ def run_requests(name, args):
     from scadyn_amd import native
     rng = np.random.default_rng(0)
     reqs = request_stream(name, rng)
-    prec = native.PREC_MIXED if args.precision == 'mixed' else native.PREC_FP64
+    prec = native.PREC_MIXED if args.precision != 'fp64' else native.PREC_FP64
     cols = [2 * ((r[4] + 1) ** 2 - 1) for r in reqs]
     if name.startswith('serving'):
         # arrival rate for ~80% utilisation from a calibration solve of the first request
@@ -92,19 +98,41 @@ def cpu_baseline(op_gpu, c, e, param, k, matvecs_per_col, budget_s=25.0):
     rng = np.random.default_rng(0)
     x = rng.standard_normal(3 * o.g.N) + 1j * rng.standard_normal(3 * o.g.N)
     n = 0
-    t0 = time.time()
-    while True:
-        x = pfft_ref.matvec(o, x)
-        x /= np.linalg.norm(x)
-        n += 1
-        if time.time() - t0 > budget_s or n >= 4:
-            break
-    per_mv = (time.time() - t0) / n
+    from threadpoolctl import threadpool_limits
+    with threadpool_limits(1):  # the scalar port on one core
+        t0 = time.time()
+        while True:
+            x = pfft_ref.matvec(o, x)
+            x /= np.linalg.norm(x)
+            n += 1
+            if time.time() - t0 > budget_s or n >= 4:
+                break
+        per_mv = (time.time() - t0) / n
     cols = 1.0 / (per_mv * matvecs_per_col)
     return {'value': cols, 'unit': 'columns/s', 'cores': 1, 'kind': 'port',
             'sample': (f'{n} oracle matvecs (numpy, 1 thread) on the full {o.g.N}-tet operator, '
                        f'{per_mv:.2f} s each, x {matvecs_per_col:.1f} matvecs/column measured on the '
                        f'GPU; operator arrays exported from the device build ({prep:.0f} s prep)')}
+
+
+def t_rel(Ta, Tb):
+    """Relative Frobenius difference of two T-matrices (four nm x nm blocks each)."""
+    num = np.sqrt(sum(np.linalg.norm(a - b) ** 2 for a, b in zip(Ta, Tb)))
+    return float(num / np.sqrt(sum(np.linalg.norm(b) ** 2 for b in Tb)))
+
+
+# HBM bytes per launch of the dominant matvec kernel, from rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE runs of `python -m scadyn_amd.probe` on the same
+# workload and batch, each scaled by the copy-kernel calibration factor:
+# (workload, kernel, precision, batch) -> bytes
+MEASURED_TRAFFIC = {
+    ('ellipsoid200k', 'gather_near', 'fp64', 112): 63.15e9,   # profiles/pmc_traffic_matvec_ellipsoid200k_b112.md
+    ('ellipsoid200k', 'gather_near', 'mixed', 112): 61.27e9,
+}
+
+
+def measured_traffic(workload, kernel, precision, batch):
+    return MEASURED_TRAFFIC.get((workload, kernel, precision, batch))
 
 
 def compare(args, c, e, param, k, nmax, desc):
@@ -131,9 +159,7 @@ def compare(args, c, e, param, k, nmax, desc):
         print(f"[bench] {name}: {el:.2f} s, {ncol / el:.3f} columns/s, mean iterations {it.mean():.1f}, "
               f"max restart cycles {inf['outer_rounds']}", file=sys.stderr, flush=True)
         op.close()
-    num = np.sqrt(sum(np.linalg.norm(a - b) ** 2 for a, b in zip(res['mixed']['T'], res['fp64']['T'])))
-    den = np.sqrt(sum(np.linalg.norm(b) ** 2 for b in res['fp64']['T']))
-    err = float(num / den)
+    err = t_rel(res['mixed']['T'], res['fp64']['T'])
     for r in res.values():
         r.pop('T')
     out = {'metric': METRIC, 'value': res['mixed']['columns_per_s'], 'unit': 'columns/s', 'n_gpus': 1,
@@ -160,9 +186,11 @@ def main():
     ap.add_argument('--workload', default='ellipsoid200k')
     ap.add_argument('--batch', type=int, default=0)
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--precision', default='fp64', choices=['fp64', 'mixed'])
+    ap.add_argument('--precision', default='both', choices=['fp64', 'mixed', 'both'],
+                    help='both: the FP64 solve (the reported value) and the mixed-precision one, with '
+                         'their ratio and the relative difference of the two T-matrices')
     ap.add_argument('--tol', type=float, default=1e-5)
-    ap.add_argument('--refine-tol', type=float, default=1e-9)
+    ap.add_argument('--refine-tol', type=float, default=1e-9, help='request loop: mixed-precision target')
     ap.add_argument('--max-active', type=int, default=8)
     ap.add_argument('--ortho', type=int, default=0, help='0 MGS (reference), 1 CGS + DGKS, 2 block MGS')
     ap.add_argument('--maxit', type=int, default=50, help='Krylov dimension per cycle (reference default 50)')
@@ -186,7 +214,7 @@ def main():
         out = {'metric': METRIC, 'value': res['columns_per_s'], 'unit': 'columns/s', 'n_gpus': 1,
                'steps': 1, 'warmup': 0, 'ms_per_step': 1000.0 * res['makespan_s'], 'higher_is_better': True,
                'scaling': 'weak', 'vs_baseline': None,
-               'dtype': 'f32 matvec + f64 refinement' if args.precision == 'mixed' else 'f64',
+               'dtype': DTYPES['fp64' if args.precision == 'fp64' else 'mixed'],
                'data': 'synthetic (seeded mesh generator)', 'config': dict(workload=args.workload, **res)}
         print(json.dumps(out), flush=True)
         return
@@ -198,18 +226,53 @@ def main():
     if args.compare:
         compare(args, c, e, param, k, nmax, desc)
         return
+    legs = ['fp64', 'mixed'] if args.precision == 'both' else [args.precision]
+    runs, cpu = {}, None
+    for name in legs:
+        r = runs[name] = measure(args, name, c, e, param, k, nmax)
+        if name == legs[0] and not args.no_cpu:
+            cpu = cpu_baseline(r['op'], c, e, param, k, r['matvecs_per_column'])
+            print(f"[bench] cpu baseline {cpu['value']:.4g} columns/s", file=sys.stderr, flush=True)
+        r.pop('op').close()  # one operator resident at a time
+    head = runs[legs[0]]
+    ncol = head['columns']
+    cfg = {'workload': desc, 'n_tet': int(e.shape[0]), 'columns': ncol, 'nmax': nmax,
+           'grid': head['grid'], 'fft': head['fft'], 'near_blocks': head['near_blocks'],
+           'gmres': {'tol': args.tol, 'maxit': args.maxit, 'restart': args.restart,
+                     'ortho': ORTHO_NAMES[args.ortho]}}
+    for name, r in runs.items():
+        cfg[name] = {kk: v for kk, v in r.items() if kk not in ('T', 'roofline', 'kernels', 'assembly_gemm')}
+        cfg[name]['kernels'] = r['kernels']
+    if len(runs) == 2:
+        cfg['speedup_mixed_over_fp64'] = runs['mixed']['columns_per_s'] / runs['fp64']['columns_per_s']
+        cfg['T_rel_frobenius_mixed_vs_fp64'] = t_rel(runs['mixed']['T'], runs['fp64']['T'])
+    out = {
+        'metric': METRIC, 'value': head['columns_per_s'], 'unit': 'columns/s', 'n_gpus': 1,
+        'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': head['ms_per_step'],
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': DTYPES[legs[0]], 'data': 'synthetic (seeded mesh generator)', 'config': cfg,
+        'roofline': head['roofline'], 'cpu_baseline': cpu, 'assembly_gemm': head['assembly_gemm'],
+    }
+    print(json.dumps(out), flush=True)
+
+
+def measure(args, name, c, e, param, k, nmax):
+    """Operator setup, then --warmup untimed and --steps timed T-matrix solves in
+    one precision; per-kernel timings from the library's HIP-event profiler."""
+    from scadyn_amd import native
     t0 = time.time()
-    prec = native.PREC_MIXED if args.precision == 'mixed' else native.PREC_FP64
+    prec = native.PREC_MIXED if name == 'mixed' else native.PREC_FP64
+    # mixed: a restart cycle ends once the Arnoldi estimate has fallen by --tol,
+    # and a system is done when its FP64 true residual is <= --tol ||b||, the
+    # criterion of the FP64 GMRES
     op = native.Operator(c, e, param, k, batch=args.batch, precision=prec, tol=args.tol,
-                         refine_tol=args.refine_tol, ortho=args.ortho, maxit=args.maxit, restart=args.restart)
+                         refine_tol=args.tol, ortho=args.ortho, maxit=args.maxit, restart=args.restart)
     setup_s = time.time() - t0
-    info = op.info()
-    print(f'[bench] setup {setup_s:.2f} s: {info}', file=sys.stderr, flush=True)
     ncol = 2 * ((nmax + 1) ** 2 - 1)
     for w in range(args.warmup):
         t = time.time()
         _, it = op.tmatrix(nmax)
-        print(f'[bench] warmup {w}: {time.time() - t:.2f} s, mean iters {it.mean():.1f}', file=sys.stderr,
+        print(f'[bench] {name} warmup {w}: {time.time() - t:.2f} s, mean iters {it.mean():.1f}', file=sys.stderr,
               flush=True)
     op.profile(True)
     t0 = time.time()
@@ -217,12 +280,11 @@ def main():
     for s in range(args.steps):
         T, it = op.tmatrix(nmax)
         iters.append(it)
-        print(f'[bench] step {s}: {time.time() - t0:.2f} s cumulative', file=sys.stderr, flush=True)
+        print(f'[bench] {name} step {s}: {time.time() - t0:.2f} s cumulative', file=sys.stderr, flush=True)
     elapsed = time.time() - t0
     tim = op.timings()
     op.profile(False)
-    ms_per_step = 1000.0 * elapsed / args.steps
-    value = ncol * args.steps / elapsed
+    info = op.info()
     it = np.concatenate(iters)
     # matvecs per column: `ite` counts unconverged inner iterations; add the
     # converged one and one restart matvec per cycle
@@ -247,31 +309,26 @@ def main():
                 'achieved': tflops, 'peak': FP64_MFMA_PEAK_TFLOPS, 'unit': 'TFLOP/s',
                 'frac': tflops / FP64_MFMA_PEAK_TFLOPS}
         per_kernel['tgemm'] = gemm
-    print('[bench] kernels ' + json.dumps(per_kernel), file=sys.stderr, flush=True)
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(op, c, e, param, k, mv_per_col)
-        print(f"[bench] cpu baseline {cpu['value']:.4g} columns/s", file=sys.stderr, flush=True)
-    out = {
-        'metric': METRIC, 'value': value, 'unit': 'columns/s', 'n_gpus': 1, 'steps': args.steps,
-        'warmup': args.warmup, 'ms_per_step': ms_per_step, 'higher_is_better': True, 'scaling': 'weak',
-        'vs_baseline': None, 'dtype': 'f64', 'data': 'synthetic (seeded mesh generator)',
-        'config': {'workload': desc, 'n_tet': int(e.shape[0]), 'columns': ncol, 'nmax': nmax,
-                   'grid': [info['Nx'], info['Ny'], info['Nz']], 'fft': [info['Lx'], info['Ly'], info['Lz']],
-                   'near_blocks': info['n_blocks'], 'batch': info['batch'], 'setup_s': setup_s,
-                   'mean_gmres_iters': float(it.mean()), 'matvecs_per_column': mv_per_col,
-                   'gmres': {'tol': args.tol, 'maxit': args.maxit, 'restart': args.restart,
-                             'ortho': ORTHO_NAMES[args.ortho],
-                             'columns_at_iteration_cap': int(np.sum(it >= args.maxit * args.restart))},
-                   'columns_per_s_with_setup': ncol * args.steps / (elapsed + setup_s * args.steps)},
-        'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
+    print(f'[bench] {name} kernels ' + json.dumps(per_kernel), file=sys.stderr, flush=True)
+    value = ncol * args.steps / elapsed
+    print(f"[bench] {name}: {value:.3f} columns/s, mean iterations {it.mean():.1f}, batch {info['batch']}, "
+          f"fft tiles {info['fft_tiles']} / {info['fft_tiles_f32']}", file=sys.stderr, flush=True)
+    dt = 'f32' if name == 'mixed' else 'f64'
+    return {
+        'op': op, 'T': T, 'columns': ncol, 'columns_per_s': value, 'ms_per_step': 1000.0 * elapsed / args.steps,
+        'seconds': elapsed, 'setup_s': setup_s, 'columns_per_s_with_setup': ncol * args.steps / (
+            elapsed + setup_s * args.steps),
+        'grid': [info['Nx'], info['Ny'], info['Nz']], 'fft': [info['Lx'], info['Ly'], info['Lz']],
+        'near_blocks': info['n_blocks'], 'batch': info['batch'], 'fft_tiles': info['fft_tiles'],
+        'fft_tiles_f32': info['fft_tiles_f32'], 'mean_gmres_iters': float(it.mean()),
+        'matvecs_per_column': mv_per_col, 'max_restart_cycles': info['outer_rounds'],
+        'columns_at_iteration_cap': int(np.sum(it >= args.maxit * args.restart)),
+        'kernels': per_kernel, 'assembly_gemm': gemm,
+        'roofline': {'bound': 'hbm', 'kernel': dom, 'dtype': dt, 'achieved': achieved, 'peak': HBM_PEAK_GBS,
+                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
+                     'traffic': measured_traffic(args.workload, dom, name, info['batch']),
                      'avg_launch_us': avg_ms * 1000.0, 'bytes_per_launch': bytes_per_launch},
-        'cpu_baseline': cpu,
-        'assembly_gemm': gemm,
     }
-    print(json.dumps(out), flush=True)
-    op.close()
 
 
 if __name__ == '__main__':

@@ -121,7 +121,9 @@ void scadyn_op_destroy(scadyn_operator *op);
 
 /* Sizes: info[0..15] = N_tet, Nx, Ny, Nz, N_cubes, N_tet_cube, n_blocks,
  * Lx, Ly, Lz, batch, device bytes, most restart cycles of a system in the last
- * solve, 1 if the near-zone rows are in cube order (per-cube gather), 0... ;
+ * solve, 1 if the near-zone rows are in cube order (per-cube gather), then the
+ * tile widths of the five line-FFT passes of the current batch in FP64 and in
+ * FP32 (8 bits each: x, y, z, inverse y, inverse x; tuned for large grids);
  * delta, box_delta and k in dinfo[0..2]. */
 int scadyn_op_info(const scadyn_operator *op, int64_t info[16], double dinfo[4]);
 

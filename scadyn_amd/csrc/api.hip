@@ -402,6 +402,8 @@ static void make_matvec_space(scadyn_operator *op, int B, MatvecSpace &ms) {
     }
 }
 
+static void tune_fft_tiles(scadyn_operator *op, MatvecSpace &ms);
+
 static Workspace &workspace(scadyn_operator *op, int B) {
     const int m = op->opts.maxit;
     if (op->ws && op->ws->B == B && op->ws->m == m) return *op->ws;
@@ -412,6 +414,7 @@ static Workspace &workspace(scadyn_operator *op, int B) {
     w.n = 3LL * op->g.N;
     const size_t nB = (size_t)w.n * B;
     make_matvec_space(op, B, w.mv);
+    tune_fft_tiles(op, w.mv);
     w.V = w.mem.alloc<cd>((size_t)(m + 2) * nB);
     w.Rhs = w.mem.alloc<cd>(nB);
     w.W = w.mem.alloc<cd>(nB);
@@ -558,6 +561,76 @@ static void run_matvec(scadyn_operator *op, MatvecSpace &w, const cd *xin, const
                      c * (Nx * Ny * Nz * nB + 27.0 * 4 * g.N) + 16.0 * 6 * g.N * w.B + 80.0 * op->nblocks);
         launch_gather_near<C>(a, op->s);
     }
+}
+
+// Autotuner of the line-FFT tiles.  For every register pass of a large batch
+// (and both precisions when the operator is mixed), each candidate tile width
+// (4 to 32 inner columns per workgroup) is timed on the batch's own grids and
+// the fastest kept; the planner's default wins ties within 3%.  Small grids
+// keep the default (tuning would cost more than it saves).
+template <typename C>
+static void run_pass(scadyn_operator *op, MatvecSpace &ms, int q, const LinePass &p) {
+    C *g0 = (C *)ms.grid0, *g1 = (C *)ms.grid1;
+    const C *tw = OpData<C>::tw(op);
+    const C *twq = q == 0 || q == 4 ? tw : (q == 1 || q == 3 ? tw + op->L[0] : tw + op->L[0] + op->L[1]);
+    if (q == 2) {
+        ConvInfo ci;
+        ci.Lx = op->L[0];
+        ci.Ly = op->L[1];
+        ci.Lz = op->L[2];
+        ci.Hx = op->Hh[0];
+        ci.Hy = op->Hh[1];
+        ci.nb = 4LL * ms.B;
+        launch_line_conv<C>(p, g0, g0, twq, OpData<C>::fg(op), ci, op->s);  // in place, as in run_matvec
+    } else {
+        launch_line_fft<C>(p, g0, g1, twq, q < 2 ? -1 : +1, op->s);
+    }
+}
+
+static void tune_fft_tiles(scadyn_operator *op, MatvecSpace &ms) {
+    const size_t gsz = (size_t)op->L[0] * op->L[1] * op->g.Nz * 4 * ms.B;
+    if (op->opts.fft_kernel != 0 || gsz < ((size_t)1 << 24)) return;
+    hipStream_t s = op->s;
+    HIP_CHECK(hipMemsetAsync(ms.grid0, 0, sizeof(cd) * gsz, s));
+    HIP_CHECK(hipMemsetAsync(ms.grid1, 0, sizeof(cd) * gsz, s));
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    const size_t budget = 64 * 1024;  // same LDS budget as the planner
+    const int nprec = op->opts.precision == SCADYN_PREC_MIXED ? 2 : 1;
+    for (int pr = 0; pr < nprec; ++pr) {
+        if (pr == 1 && !op->d_fgf) break;  // no complex64 operator data yet
+        const size_t el = pr == 0 ? sizeof(cd) : sizeof(cf);
+        for (int q = 0; q < 5; ++q) {
+            LinePass &p = ms.pass[pr][q];
+            if (!p.reg) continue;
+            LinePass keep = p;
+            float best = 0.0f;
+            const int cand[] = {p.ti, 4, 8, 16, 32};
+            for (int ci = 0; ci < 5; ++ci) {
+                LinePass c = p;
+                if (ci > 0 && (cand[ci] == p.ti || !set_line_tile(c, cand[ci], budget, el))) continue;
+                for (int rep = 0; rep < 3; ++rep) {  // first run warms up
+                    if (rep == 1) HIP_CHECK(hipEventRecord(e0, s));
+                    if (pr == 0)
+                        run_pass<cd>(op, ms, q, c);
+                    else
+                        run_pass<cf>(op, ms, q, c);
+                }
+                HIP_CHECK(hipEventRecord(e1, s));
+                HIP_CHECK(hipEventSynchronize(e1));
+                float t = 0.0f;
+                HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+                if (ci == 0 || t < 0.97f * best) {
+                    best = t;
+                    keep = c;
+                }
+            }
+            p = keep;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
 }
 
 // pack/unpack between host order [b][3N] (reference tets) and [row][B] (internal)
@@ -1213,6 +1286,11 @@ int scadyn_op_info(const scadyn_operator *op, int64_t info[16], double dinfo[4])
         info[11] = (int64_t)(op->mem.bytes + (op->ws ? op->ws->mem.bytes : 0));
         info[12] = op->last_outer_rounds;
         info[13] = op->np.cube_rows ? 1 : 0;
+        if (op->ws)  // tile widths of the five line-FFT passes, 8 bits each (FP64, FP32)
+            for (int q = 0; q < 5; ++q) {
+                info[14] |= (int64_t)(op->ws->mv.pass[0][q].ti & 0xff) << (8 * q);
+                info[15] |= (int64_t)(op->ws->mv.pass[1][q].ti & 0xff) << (8 * q);
+            }
     }
     if (dinfo) {
         dinfo[0] = op->g.delta;

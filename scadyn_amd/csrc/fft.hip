@@ -266,6 +266,24 @@ size_t line_pass_lds(const LinePass &p, size_t elem) {
     return elem * ((size_t)p.L + (p.reg ? 1 : 2) * (size_t)p.L * p.ti);
 }
 
+// Register-kernel tile width ti: the largest power of two <= want that fits the
+// inner extent, the LDS budget and 512 threads.
+static int reg_tile(int L, int rmax, long long inner, size_t lds_budget, size_t elem, int want) {
+    int ti = want;
+    while (ti > 1 && (ti > inner || elem * ((size_t)L + (size_t)L * ti) > lds_budget || ti * rmax > 512)) ti /= 2;
+    return ti;
+}
+
+bool set_line_tile(LinePass &p, int ti, size_t lds_budget, size_t elem) {
+    if (!p.reg) return false;
+    const int rmax = std::max(p.rad[0], p.rad[1]);
+    const int t = reg_tile(p.L, rmax, p.inner, lds_budget, elem, ti);
+    if (t != ti) return false;
+    p.ti = ti;
+    p.threads = ((ti * rmax + 63) / 64) * 64;
+    return true;
+}
+
 void make_line_pass(LinePass &p, int L, int n_in, int n_out, long long inner, long long outer, size_t lds_budget,
                     size_t elem, bool allow_reg) {
     p.L = L;
@@ -283,9 +301,7 @@ void make_line_pass(LinePass &p, int L, int n_in, int n_out, long long inner, lo
         if (best) {
             const int r1 = best, r2 = L / best, rmax = std::max(r1, r2);
             // tile: 16 columns of 16-byte or 32 of 8-byte elements (256 contiguous bytes per row)
-            int ti = elem >= 16 ? 16 : 32;
-            while (ti > 1 && (ti > inner || elem * ((size_t)L + (size_t)L * ti) > lds_budget || ti * rmax > 512))
-                ti /= 2;
+            const int ti = reg_tile(L, rmax, inner, lds_budget, elem, elem >= 16 ? 16 : 32);
             p.reg = 1;
             p.nrad = 2;
             p.rad[0] = r1;

@@ -36,6 +36,9 @@ struct ConvInfo {
 void make_line_pass(LinePass &p, int L, int n_in, int n_out, long long inner, long long outer, size_t lds_budget,
                     size_t elem, bool allow_reg = true);
 bool reg_radix_ok(int r);
+// register kernel: switch the tile width to ti (a power of two); false when ti
+// does not fit the pass (then p is unchanged)
+bool set_line_tile(LinePass &p, int ti, size_t lds_budget, size_t elem);
 template <typename C>
 void launch_line_reg(const LinePass &p, const C *in, C *out, const C *tw, int sign, const C *fg, const ConvInfo &ci,
                      int mode, hipStream_t s);

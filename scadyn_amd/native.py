@@ -169,8 +169,10 @@ class Operator:
         d = np.zeros(4)
         check(lib().scadyn_op_info(self.h, i.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), _dp(d)))
         keys = ['N_tet', 'Nx', 'Ny', 'Nz', 'N_cubes', 'N_tet_cube', 'n_blocks', 'Lx', 'Ly', 'Lz',
-                'batch', 'device_bytes', 'outer_rounds', 'cube_rows']
+                'batch', 'device_bytes', 'outer_rounds', 'cube_rows', 'fft_tiles', 'fft_tiles_f32']
         out = {k: int(i[j]) for j, k in enumerate(keys)}
+        for k in ('fft_tiles', 'fft_tiles_f32'):  # x, y, z, inverse y, inverse x
+            out[k] = [(out[k] >> (8 * q)) & 0xff for q in range(5)]
         out['delta'], out['box_delta'] = float(d[0]), float(d[1])
         return out
 

@@ -38,3 +38,27 @@ def test_register_vs_stockham(n_cells, fft_pad):
         assert rel(ya[r], yb[r]) < 2e-6
     a.close()
     b.close()
+
+
+def test_tuned_tiles():
+    """A batch whose grids are large enough to be tuned (64 columns): the tile
+    widths chosen are powers of two the kernels support, and the tuned matvec
+    still equals the Stockham one."""
+    from scadyn_amd import native
+    c, e, param, k = sphere_case(n_cells=16)
+    a = native.Operator(c, e, param, k, precision=native.PREC_MIXED)
+    b = native.Operator(c, e, param, k, precision=native.PREC_MIXED, fft_kernel=1)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((64, 3 * e.shape[0])) + 1j * rng.standard_normal((64, 3 * e.shape[0]))
+    ya, yb = a.matvec(x), b.matvec(x)
+    inf = a.info()
+    say(f"{e.shape[0]} tets, batch {inf['batch']}, tiles fp64 {inf['fft_tiles']} fp32 {inf['fft_tiles_f32']}")
+    for t in inf['fft_tiles'] + inf['fft_tiles_f32']:
+        assert t in (1, 2, 4, 8, 16, 32)
+    for r in range(64):
+        assert rel(ya[r], yb[r]) < 1e-13
+    ya, yb = a.matvec(x, precision=native.PREC_MIXED), b.matvec(x, precision=native.PREC_MIXED)
+    for r in range(64):
+        assert rel(ya[r], yb[r]) < 2e-6
+    a.close()
+    b.close()
