@@ -126,8 +126,8 @@ def t_rel(Ta, Tb):
 # workload and batch, each scaled by the copy-kernel calibration factor:
 # (workload, kernel, precision, batch) -> bytes
 MEASURED_TRAFFIC = {
-    ('ellipsoid200k', 'gather_near', 'fp64', 112): 63.15e9,   # profiles/pmc_traffic_matvec_ellipsoid200k_b112.md
-    ('ellipsoid200k', 'gather_near', 'mixed', 112): 61.27e9,
+    ('ellipsoid200k', 'gather_near', 'fp64', 112): 25.59e9,   # profiles/pmc_traffic_matvec_ellipsoid200k_b112.md
+    ('ellipsoid200k', 'gather_near', 'mixed', 112): 25.98e9,
 }
 
 

@@ -15,6 +15,13 @@
 // The rows of a cube's tets list the near tets in the same order (near cube
 // offset, then tet id), so entry q of every row of the cube is column q.
 //
+// Work order.  Item w is (slot chunk w / n_ne, cube ne_cubes[w % n_ne]): one
+// chunk sweeps all cubes before the next, and the workgroups are renumbered so
+// that each XCD runs one contiguous range of items (xcd_remap).  Neighbouring
+// cubes then run at the same time on the same XCD, and the x of a near tet that
+// one cube's wave fetches is still in that XCD's L2 when the waves of the
+// cubes beside it along x ask for it.
+//
 // Buffers: grid [Nz][Ny][Nx][4][B], x and y [3N][B] (x at in_off[b]), S4
 // [N][27][4], row_start [N+1], ent_blk [row_start[N]], blocks [P][9] complex64.
 #pragma once
@@ -45,10 +52,24 @@ struct GatherArgs {
     cd *y;
 };
 
+// row tets per pass (8 covers all tets of nearly every cube of a cube-split
+// mesh, so the near-zone x of a cube is fetched once) and column tets per
+// staged near chunk (GT x VC = 64: one block per lane)
+constexpr int GATHER_GT = 8, GATHER_VC = 8;
+
 template <int GT, int VC>
 struct Shape {
     static constexpr int LDS_N = (GT * 108 > GT * VC * 9) ? GT * 108 : GT * VC * 9;  // staging elements
 };
+
+// Workgroup b of a G-workgroup launch -> item index, a bijection of [0, G).
+// Workgroups are dealt to the 8 XCDs round-robin (b % 8); XCD x is given the
+// x-th contiguous range of items, the first G % 8 ranges one longer.
+GC_HD long long xcd_remap(long long b, long long G) {
+    const long long per = (G + 7) / 8, tall = G % 8 == 0 ? 8 : G % 8;
+    const long long x = b % 8, j = b / 8;
+    return x < tall ? x * per + j : tall * per + (x - tall) * (per - 1) + j;
+}
 
 // one wave's item: cube and slot
 struct Item {
@@ -61,9 +82,8 @@ struct Item {
 
 template <typename C>
 GC_HD Item make_item(const GatherArgs<C> &g, long long w, int lane) {
-    const int nch = (g.B + 63) >> 6;
-    const int ci = (int)(w / nch);
-    const int braw = (int)(w - (long long)ci * nch) * 64 + lane;
+    const int ci = (int)(w % g.n_ne);
+    const int braw = (int)(w / g.n_ne) * 64 + lane;
     Item it;
     it.valid = braw < g.B;
     it.b = it.valid ? braw : g.B - 1;
@@ -84,27 +104,32 @@ GC_HD void stage_s4(const GatherArgs<C> &g, int tp, int nt, int lane, C *sm) {
     for (int e = lane; e < nt * 108; e += 64) sm[e] = g.S4[(size_t)tp * 108 + e];
 }
 
-template <typename C, int GT>
+// The steps below take the pass's row-tet count NT as a template argument
+// (the kernel switches on it), so that the accumulators are exactly NT x 3
+// registers and no loop over them carries a run-time bound.
+template <typename C, int NT>
 GC_HD void zero_acc(C (*acc)[3]) {
 #pragma unroll
-    for (int u = 0; u < GT; ++u) acc[u][0] = acc[u][1] = acc[u][2] = Prec<C>::zero();
+    for (int u = 0; u < NT; ++u) acc[u][0] = acc[u][1] = acc[u][2] = Prec<C>::zero();
 }
 
 // y_c += S_{c+1}^T g3 - k^2 S^T g_c over the 27 box nodes
-template <typename C, int GT>
-GC_HD void grid_part(const GatherArgs<C> &g, const Item &it, int nt, const C *sm, C (*acc)[3]) {
+template <typename C, int NT>
+GC_HD void grid_part(const GatherArgs<C> &g, const Item &it, const C *sm, C (*acc)[3]) {
     typedef typename Prec<C>::real R;
     const R kk = (R)g.k2;
     const int B = g.B;
+#pragma unroll 1
     for (int jz = 0; jz < 3; ++jz)
+#pragma unroll 1
         for (int jy = 0; jy < 3; ++jy)
+#pragma unroll 1
             for (int jx = 0; jx < 3; ++jx) {
                 const int j = jx + 3 * jy + 9 * jz;
                 const C *gp = g.grid + (it.base + jx + (size_t)g.Nx * (jy + (size_t)g.Ny * jz)) * 4 * B + it.b;
                 const C g0 = gp[0], g1 = gp[(size_t)B], g2 = gp[(size_t)2 * B], g3 = gp[(size_t)3 * B];
 #pragma unroll
-                for (int u = 0; u < GT; ++u) {
-                    if (u >= nt) break;
+                for (int u = 0; u < NT; ++u) {
                     const C *s = sm + u * 108 + j * 4;
                     const C s0 = cscale(s[0], -kk);
                     cfma(acc[u][0], s0, g0);
@@ -121,6 +146,7 @@ GC_HD void grid_part(const GatherArgs<C> &g, const Item &it, int nt, const C *sm
 // entry q + lane % VC of the row (rsu = row_start of the lane's row tet)
 template <typename C, int GT, int VC>
 GC_HD void stage_blocks(const GatherArgs<C> &g, int nt, int nv, int q, int lane, int64_t rsu, C *sm) {
+    static_assert(GT * VC <= 64, "one block per lane");
     const int su = lane / VC, sv = lane % VC;
     if (su >= nt || sv >= nv) return;
     const uint32_t eb = g.ent_blk[rsu + q + sv];
@@ -141,17 +167,17 @@ GC_HD int64_t lane_row_start(const GatherArgs<C> &g, int tp, int nt, int lane) {
 }
 
 // apply the staged blocks of column tets v0 .. v0+nv-1
-template <typename C, int GT, int VC>
-GC_HD void chunk_part(const GatherArgs<C> &g, const Item &it, int nt, int v0, int nv, const C *sm, C (*acc)[3]) {
+template <typename C, int NT, int VC>
+GC_HD void chunk_part(const GatherArgs<C> &g, const Item &it, int v0, int nv, const C *sm, C (*acc)[3]) {
     const int B = g.B;
     const cd *xb = g.xin + g.in_off[it.b] + it.b;
+#pragma unroll 1
     for (int v = 0; v < nv; ++v) {
         const int col = v0 + v;
         const C x0 = Prec<C>::from(xb[(size_t)(3 * col) * B]), x1 = Prec<C>::from(xb[(size_t)(3 * col + 1) * B]),
                 x2 = Prec<C>::from(xb[(size_t)(3 * col + 2) * B]);
 #pragma unroll
-        for (int u = 0; u < GT; ++u) {
-            if (u >= nt) break;
+        for (int u = 0; u < NT; ++u) {
             const C *m = sm + (u * VC + v) * 9;
             cfma(acc[u][0], m[0], x0);
             cfma(acc[u][0], m[3], x1);
@@ -166,12 +192,11 @@ GC_HD void chunk_part(const GatherArgs<C> &g, const Item &it, int nt, int v0, in
     }
 }
 
-template <typename C, int GT>
-GC_HD void write_out(const GatherArgs<C> &g, const Item &it, int tp, int nt, C (*acc)[3]) {
+template <typename C, int NT>
+GC_HD void write_out(const GatherArgs<C> &g, const Item &it, int tp, C (*acc)[3]) {
     if (!it.valid || !it.act) return;  // inactive slots are not written
 #pragma unroll
-    for (int u = 0; u < GT; ++u) {
-        if (u >= nt) break;
+    for (int u = 0; u < NT; ++u) {
         cd *yo = g.y + (size_t)(3 * (tp + u)) * g.B + it.b;
 #pragma unroll
         for (int r = 0; r < 3; ++r) yo[(size_t)r * g.B] = Prec<cd>::from(acc[u][r]);

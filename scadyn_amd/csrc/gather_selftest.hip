@@ -7,7 +7,8 @@
 //   y_t = sum_j (S_{c+1}[t][j] g3[node_j] - k^2 S[t][j] g_c[node_j])
 //       + sum_{near tets v} A(t, v) x_v
 // on a synthetic cube lattice: 0 to 11 tets per cube (empty cubes, cubes with
-// more tets than one staging pass holds), near-zone entries pointing at
+// more tets than one staging pass holds), in the kernel's workgroup order
+// (xcd_remap, itself checked to be a bijection), near-zone entries pointing at
 // shuffled blocks with and without the transpose flag, inactive slots (left
 // unwritten), and
 // batch widths 64, 112 (partial second chunk) and 56.
@@ -32,32 +33,47 @@ static C to(cd a) {
     return Prec<C>::from(a);
 }
 
+// one pass of NT row tets, lane by lane, barriers as phase boundaries
+template <typename C, int NT, int GT, int VC>
+static void emulate_pass(const GatherArgs<C> &g, const std::vector<Item> &it, int tp, std::vector<C> &sm) {
+    std::vector<std::array<std::array<C, 3>, NT>> acc(64);
+    for (int l = 0; l < 64; ++l) zero_acc<C, NT>((C(*)[3])acc[l].data());
+    for (int l = 0; l < 64; ++l) stage_s4(g, tp, NT, l, sm.data());
+    for (int l = 0; l < 64; ++l) grid_part<C, NT>(g, it[l], sm.data(), (C(*)[3])acc[l].data());
+    std::vector<int64_t> rsu(64);
+    for (int l = 0; l < 64; ++l) rsu[l] = lane_row_start<C, GT, VC>(g, tp, NT, l);
+    int q = 0;
+    for_near_cubes(g, it[0], [&](int c0, int c1) {
+        for (int v0 = c0; v0 < c1; v0 += VC) {
+            const int nv = std::min(VC, c1 - v0);
+            for (int l = 0; l < 64; ++l) stage_blocks<C, GT, VC>(g, NT, nv, q, l, rsu[l], sm.data());
+            for (int l = 0; l < 64; ++l) chunk_part<C, NT, VC>(g, it[l], v0, nv, sm.data(), (C(*)[3])acc[l].data());
+            q += nv;
+        }
+    });
+    for (int l = 0; l < 64; ++l) write_out<C, NT>(g, it[l], tp, (C(*)[3])acc[l].data());
+}
+
 template <typename C, int GT, int VC>
 static void emulate(const GatherArgs<C> &g) {
+    static_assert(GT == 8, "pass switch covers 1..8");
     std::vector<C> sm(Shape<GT, VC>::LDS_N);
-    std::vector<std::array<std::array<C, 3>, GT>> acc(64);
     const long long nw = (long long)g.n_ne * ((g.B + 63) >> 6);
-    for (long long w = 0; w < nw; ++w) {
+    for (long long b = 0; b < nw; ++b) {
+        const long long w = xcd_remap(b, nw);  // the kernel's order (one item per workgroup)
         std::vector<Item> it(64);
         for (int l = 0; l < 64; ++l) it[l] = make_item(g, w, l);
         for (int tp = it[0].t0; tp < it[0].t1; tp += GT) {
-            const int nt = std::min(GT, it[0].t1 - tp);
-            for (int l = 0; l < 64; ++l) zero_acc<C, GT>((C(*)[3])acc[l].data());
-            for (int l = 0; l < 64; ++l) stage_s4(g, tp, nt, l, sm.data());
-            for (int l = 0; l < 64; ++l) grid_part<C, GT>(g, it[l], nt, sm.data(), (C(*)[3])acc[l].data());
-            std::vector<int64_t> rsu(64);
-            for (int l = 0; l < 64; ++l) rsu[l] = lane_row_start<C, GT, VC>(g, tp, nt, l);
-            int q = 0;
-            for_near_cubes(g, it[0], [&](int c0, int c1) {
-                for (int v0 = c0; v0 < c1; v0 += VC) {
-                    const int nv = std::min(VC, c1 - v0);
-                    for (int l = 0; l < 64; ++l) stage_blocks<C, GT, VC>(g, nt, nv, q, l, rsu[l], sm.data());
-                    for (int l = 0; l < 64; ++l)
-                        chunk_part<C, GT, VC>(g, it[l], nt, v0, nv, sm.data(), (C(*)[3])acc[l].data());
-                    q += nv;
-                }
-            });
-            for (int l = 0; l < 64; ++l) write_out<C, GT>(g, it[l], tp, nt, (C(*)[3])acc[l].data());
+            switch (std::min(GT, it[0].t1 - tp)) {
+                case 1: emulate_pass<C, 1, GT, VC>(g, it, tp, sm); break;
+                case 2: emulate_pass<C, 2, GT, VC>(g, it, tp, sm); break;
+                case 3: emulate_pass<C, 3, GT, VC>(g, it, tp, sm); break;
+                case 4: emulate_pass<C, 4, GT, VC>(g, it, tp, sm); break;
+                case 5: emulate_pass<C, 5, GT, VC>(g, it, tp, sm); break;
+                case 6: emulate_pass<C, 6, GT, VC>(g, it, tp, sm); break;
+                case 7: emulate_pass<C, 7, GT, VC>(g, it, tp, sm); break;
+                default: emulate_pass<C, 8, GT, VC>(g, it, tp, sm); break;
+            }
         }
     }
 }
@@ -116,7 +132,7 @@ static void run_case(int B, std::mt19937_64 &rng) {
     GatherArgs<C> g = {grid.data(), xin.data(), in_off.data(), B, ne.data(), (int)ne.size(), cube_start.data(),
                        S4.data(), Nx, Ny, ncx, ncy, ncz, row_start.data(), ent_blk.data(), blocks.data(), k2,
                        active.data(), y.data()};
-    emulate<C, 4, 16>(g);
+    emulate<C, GATHER_GT, GATHER_VC>(g);
     // direct evaluation in double
     double num = 0.0, den = 0.0;
     for (int t = 0; t < N; ++t) {
@@ -163,7 +179,36 @@ static void run_case(int B, std::mt19937_64 &rng) {
                 sizeof(C) == 16 ? "c128" : "c64 ", B, N, ncubes, ne.size(), E, err, ok ? "ok" : "FAIL");
 }
 
+// xcd_remap is a bijection of [0, G) that gives XCD b % 8 one contiguous range
+static void check_remap() {
+    std::vector<long long> Gs;
+    for (long long G = 1; G <= 300; ++G) Gs.push_back(G);
+    for (long long G : {71864LL, 71865LL, 1048575LL, 1048576LL}) Gs.push_back(G);
+    int bad = 0;
+    for (long long G : Gs) {
+        std::vector<char> seen(G, 0);
+        std::vector<long long> lo(8, G), hi(8, -1), n(8, 0);
+        for (long long b = 0; b < G; ++b) {
+            const long long w = xcd_remap(b, G);
+            if (w < 0 || w >= G || seen[w]) {
+                ++bad;
+                break;
+            }
+            seen[w] = 1;
+            const int x = (int)(b % 8);
+            lo[x] = std::min(lo[x], w);
+            hi[x] = std::max(hi[x], w);
+            ++n[x];
+        }
+        for (int x = 0; x < 8; ++x)
+            if (n[x] && hi[x] - lo[x] + 1 != n[x]) ++bad;  // contiguous
+    }
+    std::printf("xcd_remap over %zu grid sizes: %s\n", Gs.size(), bad ? "FAIL" : "ok");
+    if (bad) ++failures;
+}
+
 int main() {
+    check_remap();
     std::mt19937_64 rng(99);
     for (int B : {64, 112, 56}) {
         run_case<cd>(B, rng);

@@ -461,35 +461,54 @@ __global__ __launch_bounds__(256) void k_gather_cube_w64(
 // gather_core.hpp (shared with the host self-test).  Compared with scalar
 // (SMEM) operand loads this keeps many operand fetches in flight and converts
 // each near-zone block once per wave instead of once per lane.
-template <typename C, int GT, int VC>
-__global__ __launch_bounds__(64, 2) void k_gather_lds(gcore::GatherArgs<C> g) {
+template <typename C, int NT, int GT, int VC>
+__device__ __forceinline__ void gather_pass(const gcore::GatherArgs<C> &g, const gcore::Item &it, int tp, int lane,
+                                            C *sm) {
     using namespace gcore;
+    C acc[NT][3];
+    zero_acc<C, NT>(acc);
+    __syncthreads();  // the previous pass's reads of sm are done
+    stage_s4(g, tp, NT, lane, sm);
+    __syncthreads();
+    grid_part<C, NT>(g, it, sm, acc);
+    const int64_t rsu = lane_row_start<C, GT, VC>(g, tp, NT, lane);
+    int q = 0;
+    for_near_cubes(g, it, [&](int c0, int c1) {
+        for (int v0 = c0; v0 < c1; v0 += VC) {
+            const int nv = min(VC, c1 - v0);
+            __syncthreads();
+            stage_blocks<C, GT, VC>(g, NT, nv, q, lane, rsu, sm);
+            __syncthreads();
+            chunk_part<C, NT, VC>(g, it, v0, nv, sm, acc);
+            q += nv;
+        }
+    });
+    write_out<C, NT>(g, it, tp, acc);
+}
+
+template <typename C, int GT, int VC>
+__global__ __launch_bounds__(64, 3) void k_gather_lds(gcore::GatherArgs<C> g) {
+    using namespace gcore;
+    static_assert(GT == 8, "the pass switch below covers 1..8 row tets");
     __shared__ C sm[Shape<GT, VC>::LDS_N];
     const int lane = threadIdx.x;
     const long long nw = (long long)g.n_ne * ((g.B + 63) >> 6);
-    for (long long w = blockIdx.x; w < nw; w += gridDim.x) {
+    for (long long b = blockIdx.x; b < nw; b += gridDim.x) {
+        // one item per workgroup when the grid covers all items (the launch
+        // below); a grid-stride remainder keeps plain order
+        const long long w = nw == gridDim.x ? xcd_remap(b, nw) : b;
         const Item it = make_item(g, w, lane);
         for (int tp = it.t0; tp < it.t1; tp += GT) {
-            const int nt = min(GT, it.t1 - tp);
-            C acc[GT][3];
-            zero_acc<C, GT>(acc);
-            __syncthreads();
-            stage_s4(g, tp, nt, lane, sm);
-            __syncthreads();
-            grid_part<C, GT>(g, it, nt, sm, acc);
-            const int64_t rsu = lane_row_start<C, GT, VC>(g, tp, nt, lane);
-            int q = 0;
-            for_near_cubes(g, it, [&](int c0, int c1) {
-                for (int v0 = c0; v0 < c1; v0 += VC) {
-                    const int nv = min(VC, c1 - v0);
-                    __syncthreads();
-                    stage_blocks<C, GT, VC>(g, nt, nv, q, lane, rsu, sm);
-                    __syncthreads();
-                    chunk_part<C, GT, VC>(g, it, nt, v0, nv, sm, acc);
-                    q += nv;
-                }
-            });
-            write_out<C, GT>(g, it, tp, nt, acc);
+            switch (min(GT, it.t1 - tp)) {  // wave-uniform
+                case 1: gather_pass<C, 1, GT, VC>(g, it, tp, lane, sm); break;
+                case 2: gather_pass<C, 2, GT, VC>(g, it, tp, lane, sm); break;
+                case 3: gather_pass<C, 3, GT, VC>(g, it, tp, lane, sm); break;
+                case 4: gather_pass<C, 4, GT, VC>(g, it, tp, lane, sm); break;
+                case 5: gather_pass<C, 5, GT, VC>(g, it, tp, lane, sm); break;
+                case 6: gather_pass<C, 6, GT, VC>(g, it, tp, lane, sm); break;
+                case 7: gather_pass<C, 7, GT, VC>(g, it, tp, lane, sm); break;
+                default: gather_pass<C, 8, GT, VC>(g, it, tp, lane, sm); break;
+            }
         }
     }
 }
@@ -526,7 +545,7 @@ void launch_gather_near(const MatvecArgs<C> &a, hipStream_t s) {
                                             a.n_ne,     a.cube_start, a.S4,      a.Nx,     a.Ny,
                                             a.ncx,      a.ncy,       a.ncz,       a.row_start, a.ent_blk,
                                             a.blocks,   a.k2,        a.slot_active, a.y};
-            hipLaunchKernelGGL((k_gather_lds<C, 4, 16>), dim3(grid_for(nw, 1)), dim3(64), 0, s, g);
+            hipLaunchKernelGGL((k_gather_lds<C, gcore::GATHER_GT, gcore::GATHER_VC>), dim3(grid_for(nw, 1)), dim3(64), 0, s, g);
         } else {
             hipLaunchKernelGGL((k_gather_cube_w64<C, 8>), dim3(grid_for(nw, 4)), dim3(256), 0, s, a.grid_out, a.xin,
                                a.in_off, a.B, a.ne_cubes, a.n_ne, a.cube_start, a.S4, a.Nx, a.Ny, a.ncx, a.ncy,
