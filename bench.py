@@ -193,6 +193,7 @@ def main():
     ap.add_argument('--refine-tol', type=float, default=1e-9, help='request loop: mixed-precision target')
     ap.add_argument('--max-active', type=int, default=8)
     ap.add_argument('--ortho', type=int, default=0, help='0 MGS (reference), 1 CGS + DGKS, 2 block MGS')
+    ap.add_argument('--mgs-block', type=int, default=0, help='MGS coefficients per pass (0 automatic, 1, 2, 4, 8)')
     ap.add_argument('--maxit', type=int, default=50, help='Krylov dimension per cycle (reference default 50)')
     ap.add_argument('--restart', type=int, default=4, help='restart cycles (reference default 4)')
     ap.add_argument('--compare', action='store_true',
@@ -239,7 +240,7 @@ def main():
     cfg = {'workload': desc, 'n_tet': int(e.shape[0]), 'columns': ncol, 'nmax': nmax,
            'grid': head['grid'], 'fft': head['fft'], 'near_blocks': head['near_blocks'],
            'gmres': {'tol': args.tol, 'maxit': args.maxit, 'restart': args.restart,
-                     'ortho': ORTHO_NAMES[args.ortho]}}
+                     'ortho': ORTHO_NAMES[args.ortho], 'mgs_block': args.mgs_block}}
     for name, r in runs.items():
         cfg[name] = {kk: v for kk, v in r.items() if kk not in ('T', 'roofline', 'kernels', 'assembly_gemm')}
         cfg[name]['kernels'] = r['kernels']
@@ -266,7 +267,8 @@ def measure(args, name, c, e, param, k, nmax):
     # and a system is done when its FP64 true residual is <= --tol ||b||, the
     # criterion of the FP64 GMRES
     op = native.Operator(c, e, param, k, batch=args.batch, precision=prec, tol=args.tol,
-                         refine_tol=args.tol, ortho=args.ortho, maxit=args.maxit, restart=args.restart)
+                         refine_tol=args.tol, ortho=args.ortho, mgs_block=args.mgs_block, maxit=args.maxit,
+                         restart=args.restart)
     setup_s = time.time() - t0
     ncol = 2 * ((nmax + 1) ** 2 - 1)
     for w in range(args.warmup):

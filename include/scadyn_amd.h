@@ -76,8 +76,8 @@ typedef struct scadyn_opts {
                             steps use the FP32 matvec and tol is then the residual
                             reduction that ends a restart cycle early               */
     int32_t ortho;       /* Arnoldi orthogonalisation: 0 modified Gram-Schmidt (the
-                            reference's, src/gmres_module.f90:241-244), one pass per
-                            basis vector; 1 classical Gram-Schmidt with DGKS
+                            reference's, src/gmres_module.f90:241-244), mgs_block basis
+                            vectors per pass; 1 classical Gram-Schmidt with DGKS
                             re-orthogonalisation (two fused passes per iteration);
                             2 block MGS: the MGS coefficients 8 basis vectors per pass
                             with an in-pass Gram correction (same recurrence, about
@@ -85,7 +85,11 @@ typedef struct scadyn_opts {
     int32_t fft_kernel;  /* line-FFT kernels: 0 automatic (register two-stage kernel where
                             the axis length is R1*R2 with R1, R2 <= 16, else LDS
                             Stockham), 1 LDS Stockham on every axis                     */
-    int32_t reserved[6];
+    int32_t mgs_block;   /* ortho 0: basis vectors whose MGS coefficients one pass over the
+                            rows evaluates (in-pass Gram correction, same recurrence):
+                            0 automatic (4), 1, 2, 4 or 8; a pass moves about
+                            2 + 2/mgs_block vectors of HBM traffic per coefficient      */
+    int32_t reserved[5];
 } scadyn_opts;
 
 /* Defaults of the reference (src/common.f90:78-89). */

@@ -77,7 +77,8 @@ def build(verbose=False):
 
 SELFTEST = os.path.join(OBJDIR, 'fft_selftest')
 SELFTESTS = {'fft': ('fft_selftest.hip', SELFTEST), 'bmgs': ('bmgs_selftest.hip', os.path.join(OBJDIR, 'bmgs_selftest')),
-             'gather': ('gather_selftest.hip', os.path.join(OBJDIR, 'gather_selftest'))}
+             'gather': ('gather_selftest.hip', os.path.join(OBJDIR, 'gather_selftest')),
+             'mgsb': ('mgsb_selftest.hip', os.path.join(OBJDIR, 'mgsb_selftest'))}
 
 
 def build_selftest(verbose=False, which='fft'):

@@ -688,12 +688,18 @@ static int auto_batch(scadyn_operator *op, int ncols) {
     return std::max(1, B);
 }
 
+// basis vectors per MGS pass when opts.mgs_block = 0: 4 moves 2.5 vectors of
+// HBM traffic per coefficient against 4 for one per pass, at 98 VGPRs (8 needs
+// 234 and runs slower at 2 waves per SIMD)
+constexpr int MGS_BLOCK_AUTO = 4;
+
 struct SolveSpec {
     int prec = SCADYN_PREC_FP64;  // matvec precision of the Arnoldi steps
     bool x0zero = false;          // x0 = 0 (correction equations) instead of x0 = rhs
     double tol = 1e-5;
     int restart = 4;
     int ortho = 0;                // 0 MGS (reference), 1 CGS + DGKS, 2 block MGS
+    int mgs_block = 1;            // MGS: basis vectors per pass (1, 2, 4 or 8)
     // mixed precision: every restart takes the true residual with the FP64
     // matvec and a system is done only when it is <= tol_true ||b||
     bool verify64 = false;
@@ -843,6 +849,9 @@ struct ColumnSolver {
             } else if (spec.ortho == 2) {
                 launch_bmgs(d, w->W, w->V, w->Rhs, w->st, w->H, w->pdot, w->pnrm, w->pnav, w->cc, w->nrm2, w->nav2,
                             std::max(K - 1, 0), s);
+            } else if (spec.mgs_block > 1) {
+                launch_mgsb(d, spec.mgs_block, K, w->W, w->V, w->Rhs, w->st, w->H, w->pdot, w->pnrm, w->pnav,
+                            w->nrm2, w->nav2, s);
             } else {
                 for (int kk = 0; kk <= K; ++kk) {
                     launch_mgs_pass(d, kk, w->W, w->V, w->Rhs, w->st, w->H, w->pdot, w->pnrm, w->pnav, s);
@@ -895,6 +904,7 @@ static SolveSpec solve_spec(const scadyn_operator *op) {
     spec.tol = op->opts.tol;
     spec.restart = op->opts.restart;
     spec.ortho = op->opts.ortho;
+    spec.mgs_block = op->opts.mgs_block == 0 ? MGS_BLOCK_AUTO : op->opts.mgs_block;
     if (op->opts.precision == SCADYN_PREC_MIXED) {
         spec.prec = SCADYN_PREC_MIXED;
         spec.verify64 = true;
@@ -1254,6 +1264,8 @@ int scadyn_op_create(const scadyn_mesh *mesh, const scadyn_opts *opts, double k,
         fail(SCADYN_ERR_UNSUPPORTED, "block MGS (ortho = 2) is disabled on the device: its first run faulted and "
                                      "the cause is not yet found");
     if (o.fft_kernel != 0 && o.fft_kernel != 1) fail(SCADYN_ERR_ARG, "fft_kernel must be 0 (automatic) or 1 (Stockham)");
+    if (o.mgs_block != 0 && o.mgs_block != 1 && o.mgs_block != 2 && o.mgs_block != 4 && o.mgs_block != 8)
+        fail(SCADYN_ERR_ARG, "mgs_block must be 0 (automatic), 1, 2, 4 or 8");
     *out = nullptr;
     std::unique_ptr<scadyn_operator> op(new scadyn_operator());
     create_operator(op.get(), *mesh, o, k);

@@ -3,6 +3,7 @@
 
 #include "krylov.hpp"
 #include "bmgs_core.hpp"
+#include "mgsb_core.hpp"
 
 namespace scadyn {
 
@@ -142,6 +143,97 @@ __global__ __launch_bounds__(RED_T) void k_mgs_reduce(KrylovDims d, int k, const
         if (k == 0) nav2[b] = x2;
     }
 }
+
+// ---- MGS evaluated BS basis vectors per pass (opts.ortho = 0, mgs_block > 1) ----
+// The per-thread row sweep and the per-slot coefficient step are in
+// mgsb_core.hpp (shared with the host self-test).
+template <int BS>
+__global__ __launch_bounds__(256) void k_mgsb_pass(KrylovDims d, int p, cd *__restrict__ W,
+                                                   const cd *__restrict__ V, const cd *__restrict__ Rhs,
+                                                   const SlotState *__restrict__ st, const cd *__restrict__ H,
+                                                   cd *__restrict__ pdot, double *__restrict__ pnrm,
+                                                   double *__restrict__ pnav) {
+    constexpr int NQ = mgsb::Blk<BS>::NQ;
+    __shared__ cd sdot[256];
+    __shared__ double snrm[256], snav[256];
+    const RowSlotMap m = row_slot_map(d.B);
+    cd acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] = cmk(0.0, 0.0);
+    double nrm = 0.0, nav = 0.0;
+    if (m.ok)
+        mgsb::sweep_rows<BS>(d, p, W, V, Rhs, st, H, m.b, (long long)blockIdx.x * m.RW + m.rl,
+                             (long long)gridDim.x * m.RW, acc, nrm, nav);
+    snrm[threadIdx.x] = nrm;
+    snav[threadIdx.x] = nav;
+    __syncthreads();
+    if (m.ok && m.rl == 0) {
+        double a = 0.0, c = 0.0;
+        for (int r = 0; r < m.RW; ++r) {
+            a += snrm[r * m.SB + m.sl];
+            c += snav[r * m.SB + m.sl];
+        }
+        pnrm[(size_t)blockIdx.x * d.B + m.b] = a;
+        pnav[(size_t)blockIdx.x * d.B + m.b] = c;
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        __syncthreads();  // the previous partial's reads of sdot are done
+        sdot[threadIdx.x] = acc[q];
+        __syncthreads();
+        if (m.ok && m.rl == 0) {
+            cd a = cmk(0.0, 0.0);
+            for (int r = 0; r < m.RW; ++r) a = cadd(a, sdot[r * m.SB + m.sl]);
+            pdot[((size_t)blockIdx.x * NQ + q) * d.B + m.b] = a;
+        }
+    }
+}
+
+// one workgroup per slot: sums the pass's partials in a fixed order, then the
+// block's coefficients (forward substitution) or the norms
+template <int BS>
+__global__ __launch_bounds__(RED_T) void k_mgsb_reduce(KrylovDims d, int p, const SlotState *__restrict__ st,
+                                                       cd *__restrict__ H, const cd *__restrict__ pdot,
+                                                       const double *__restrict__ pnrm,
+                                                       const double *__restrict__ pnav, double *__restrict__ nrm2,
+                                                       double *__restrict__ nav2) {
+    constexpr int NQ = mgsb::Blk<BS>::NQ;
+    const int b = blockIdx.x;
+    const mgsb::PassRole r = mgsb::pass_role<BS>(st[b], p);
+    if (!r.rst && !r.arn) return;  // uniform over the workgroup
+    const double x1 = (r.rst || r.nd == 0) ? block_sum<double>(pnrm, d.nwg, d.B, b) : 0.0;
+    const double x2 = (r.arn && p == 0) ? block_sum<double>(pnav, d.nwg, d.B, b) : 0.0;
+    cd red[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+        red[q] = r.nd > 0 ? block_sum<cd>(pdot, d.nwg, (size_t)NQ * d.B, (size_t)q * d.B + b) : cmk(0.0, 0.0);
+    if (threadIdx.x == 0) mgsb::slot_coef<BS>(d, p, st, red, x1, x2, H, nrm2, nav2, b);
+}
+
+template <int BS>
+static void launch_mgsb_t(const KrylovDims &d, int K, cd *W, const cd *V, const cd *Rhs, const SlotState *st,
+                          cd *H, cd *pdot, double *pnrm, double *pnav, double *nrm2, double *nav2, hipStream_t s) {
+    const int P = (K + BS - 1) / BS;  // blocks of the vectors 0 .. K-1; pass P takes the last norm
+    const int SB = d.B < 256 ? d.B : 256;
+    const dim3 grid((unsigned)d.nwg, (unsigned)((d.B + SB - 1) / SB));
+    for (int p = 0; p <= P; ++p) {
+        hipLaunchKernelGGL((k_mgsb_pass<BS>), grid, dim3(256), 0, s, d, p, W, V, Rhs, st, H, pdot, pnrm, pnav);
+        hipLaunchKernelGGL((k_mgsb_reduce<BS>), dim3((unsigned)d.B), dim3(RED_T), 0, s, d, p, st, H, pdot, pnrm,
+                           pnav, nrm2, nav2);
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_mgsb(const KrylovDims &d, int bs, int K, cd *W, const cd *V, const cd *Rhs, const SlotState *st, cd *H,
+                 cd *pdot, double *pnrm, double *pnav, double *nrm2, double *nav2, hipStream_t s) {
+    switch (bs) {
+        case 2: launch_mgsb_t<2>(d, K, W, V, Rhs, st, H, pdot, pnrm, pnav, nrm2, nav2, s); break;
+        case 4: launch_mgsb_t<4>(d, K, W, V, Rhs, st, H, pdot, pnrm, pnav, nrm2, nav2, s); break;
+        case 8: launch_mgsb_t<8>(d, K, W, V, Rhs, st, H, pdot, pnrm, pnav, nrm2, nav2, s); break;
+        default: fail(SCADYN_ERR_ARG, "MGS block size must be 2, 4 or 8");
+    }
+}
+int mgsb_partials(int bs) { return bs + bs * (bs - 1) / 2; }
 
 // ---- block modified Gram-Schmidt (opts.ortho = 2) ------------------------------
 // The per-thread / per-slot bodies are in bmgs_core.hpp (shared with the host

@@ -271,11 +271,16 @@ __global__ __launch_bounds__(256) void k_scatter_blk_w64(const cd *__restrict__ 
     const int nch = (B + 63) >> 6;  // last chunk may be partial: lanes >= B load slot B-1, store nothing
     // ceil(N/2) blocks per axis cover every node (nodes beyond the cubes' boxes get zeros)
     const int nbx = (Nx + 1) >> 1, nby = (Ny + 1) >> 1, nbz = (Nz + 1) >> 1;
-    const long long nw = (long long)nbx * nby * nbz * nch;
+    const long long nblk = (long long)nbx * nby * nbz, nw = nblk * nch;
     const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    for (long long w = (long long)blockIdx.x * 4 + wib; w < nw; w += (long long)gridDim.x * 4) {
-        const long long blk = w / nch;
-        const int braw = (int)(w - blk * nch) * 64 + lane;
+    // items in chunk-major order (one slot chunk sweeps all node blocks), each
+    // XCD a contiguous range of workgroups when the grid covers all items, so
+    // that node blocks adjacent along x share their tets' x in one L2
+    const long long G = gridDim.x;
+    const long long wg = G * 4 >= nw ? gcore::xcd_remap(blockIdx.x, G) : blockIdx.x;
+    for (long long w = wg * 4 + wib; w < nw; w += G * 4) {
+        const long long blk = w % nblk;
+        const int braw = (int)(w / nblk) * 64 + lane;
         const bool valid = braw < B;
         const int b = valid ? braw : B - 1;
         const int ba = (int)(blk % nbx), bb = (int)((blk / nbx) % nby), bc = (int)(blk / ((long long)nbx * nby));

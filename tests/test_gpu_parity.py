@@ -165,6 +165,51 @@ def test_solve_block_mgs(maxit):
     b.close()
 
 
+@pytest.mark.parametrize('bs', [1, 2, 4, 8])
+def test_solve_mgs_blocked_vs_oracle(bs):
+    """MGS with bs basis vectors per pass (opts.mgs_block) against the oracle's
+    GMRES at the reference's settings: identical iteration counts, solutions
+    within 1e-8 (the blocked coefficients equal MGS up to rounding)."""
+    from oracle import pfft_ref
+    from scadyn_amd import native
+    c, e, param, k = sphere_case()
+    say(f'solve x2 with mgs_block={bs}')
+    g = native.Operator(c, e, param, k, fft_pad=1, mgs_block=bs)
+    o = oracle_operator()
+    rng = np.random.default_rng(2)
+    n = 3 * o.g.N
+    b = rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n))
+    x, it = g.solve(b)
+    for r in range(2):
+        xr, ite, _ = pfft_ref.gmres(lambda v: pfft_ref.matvec(o, v), b[r].copy())
+        assert it[r] == ite
+        assert rel(x[r], xr) < 1e-8
+    g.close()
+
+
+@pytest.mark.parametrize('bs,maxit', [(2, 20), (4, 20), (4, 50), (8, 50)])
+def test_solve_mgs_blocked_deep(bs, maxit):
+    """Blocked MGS against one vector per pass on the device, at a tight
+    tolerance so that the cycles run through many blocks and restarts, with 5
+    systems in one batch (slots at different columns): iteration counts within
+    one, solutions within 1e-8 (both converged to 1e-10)."""
+    from scadyn_amd import native
+    c, e, param, k = sphere_case(n_cells=6)
+    say(f'mgs_block {bs} vs 1, maxit {maxit}')
+    a = native.Operator(c, e, param, k, tol=1e-10, maxit=maxit, restart=6, mgs_block=1)
+    b = native.Operator(c, e, param, k, tol=1e-10, maxit=maxit, restart=6, mgs_block=bs)
+    rng = np.random.default_rng(31)
+    n = 3 * e.shape[0]
+    rhs = rng.standard_normal((5, n)) + 1j * rng.standard_normal((5, n))
+    xa, ia = a.solve(rhs)
+    xb, ib = b.solve(rhs)
+    say(f'iterations {ia} vs {ib}')
+    assert np.max(np.abs(ia.astype(int) - ib.astype(int))) <= 1
+    assert max(rel(xb[r], xa[r]) for r in range(5)) < 1e-8
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize('nmax', [1, 3, 8])
 def test_project_mfma_gemm(nmax):
     """mat^H X on the FP64 matrix cores vs numpy, at 2nm = 6, 30 and 160 columns
