@@ -46,7 +46,9 @@ extern "C" {
 #define SCADYN_ERR_UNSUPPORTED  8  /* option the build does not implement            */
 
 #define SCADYN_PREC_FP64   0  /* FP64 matvec, reference-faithful GMRES          */
-#define SCADYN_PREC_MIXED  1  /* FP32 matvec inside FP64 iterative refinement   */
+#define SCADYN_PREC_MIXED  1  /* FP32 matvec (and, with the blocked MGS, an FP32-stored
+                                 Krylov basis with FP64 arithmetic) inside FP64 iterative
+                                 refinement                                          */
 
 typedef struct scadyn_mesh {
     int32_t n_node;

@@ -140,7 +140,15 @@ struct Workspace {
     int rs_cap = 0;
     cd *rx = nullptr, *ry = nullptr;
     int *rslot = nullptr;
-    cd *V = nullptr, *Rhs = nullptr, *W = nullptr;
+    // Krylov basis: complex128 V [(m+2)][n][B] with x in block m+1 (X points
+    // there), or for the mixed-precision solve (ortho 0, mgs_block > 1) a
+    // complex64 basis Vf [(m+1)][n][B] and x in its own complex128 X
+    cd *V = nullptr, *Rhs = nullptr, *W = nullptr, *X = nullptr;
+    cf *Vf = nullptr;
+    // mixed precision with a complex128 basis: the batch's inputs converted to
+    // complex64 (Xs [n][B], read with zero offsets zoff)
+    cf *Xs = nullptr;
+    long long *zoff = nullptr;
     SlotState *st = nullptr;
     cd *H = nullptr, *cs = nullptr, *sn = nullptr, *g = nullptr, *y = nullptr;
     double *nrm2 = nullptr, *nav2 = nullptr, *pnrm = nullptr, *pnav = nullptr;
@@ -211,6 +219,39 @@ static void fft3d_full(scadyn_operator *op, cd *A, cd *tmp) {
     make_line_pass(p, Lz, Lz, Lz, (long long)Lx * Ly, 1, budget, sizeof(cd), reg);
     launch_line_fft<cd>(p, A, tmp, op->d_tw + Lx + Ly, -1, op->s);
     HIP_CHECK(hipMemcpyAsync(A, tmp, sizeof(cd) * (size_t)Lx * Ly * Lz, hipMemcpyDeviceToDevice, op->s));
+}
+
+// basis vectors per MGS pass when opts.mgs_block = 0: 4 moves 2.5 vectors of
+// HBM traffic per coefficient against 4 for one per pass, at 98 VGPRs (8 needs
+// 234 and runs slower at 2 waves per SIMD)
+constexpr int MGS_BLOCK_AUTO = 4;
+
+// The mixed-precision solve keeps its Krylov basis in complex64 (arithmetic
+// stays FP64) when the blocked MGS runs: the basis is what its sweeps and the
+// complex64 matvec read.
+static bool f32_basis(const scadyn_operator *op) {
+    const int bs = op->opts.mgs_block == 0 ? MGS_BLOCK_AUTO : op->opts.mgs_block;
+    return op->opts.precision == SCADYN_PREC_MIXED && op->opts.ortho == 0 && bs > 1;
+}
+
+// xs[row*B + b] = (complex64) v[in_off[b] + row*B + b] for active slots
+__global__ void k_stage_f32(const cd *__restrict__ v, const long long *__restrict__ in_off,
+                            const int *__restrict__ active, long long n, int B, cf *__restrict__ xs) {
+    const long long tot = n * B;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int b = (int)(e % B);
+        if (!active[b]) continue;
+        const cd a = v[in_off[b] + e];
+        xs[e] = make_float2((float)a.x, (float)a.y);
+    }
+}
+static void launch_stage_f32(const cd *v, const long long *in_off, const int *active, long long n, int B, cf *xs,
+                             hipStream_t s) {
+    const long long tot = n * B;
+    const unsigned g = (unsigned)std::min<long long>((tot + 255) / 256, 1 << 16);
+    hipLaunchKernelGGL(k_stage_f32, dim3(g), dim3(256), 0, s, v, in_off, active, n, B, xs);
+    HIP_CHECK(hipGetLastError());
 }
 
 __global__ void k_to_f32(const cd *__restrict__ a, cf *__restrict__ b, size_t n) {
@@ -415,7 +456,18 @@ static Workspace &workspace(scadyn_operator *op, int B) {
     const size_t nB = (size_t)w.n * B;
     make_matvec_space(op, B, w.mv);
     tune_fft_tiles(op, w.mv);
-    w.V = w.mem.alloc<cd>((size_t)(m + 2) * nB);
+    if (f32_basis(op)) {
+        w.Vf = w.mem.alloc<cf>((size_t)(m + 1) * nB);
+        w.X = w.mem.alloc<cd>(nB);
+    } else {
+        w.V = w.mem.alloc<cd>((size_t)(m + 2) * nB);
+        w.X = w.V + (size_t)(m + 1) * nB;
+        if (op->opts.precision == SCADYN_PREC_MIXED) {
+            w.Xs = w.mem.alloc<cf>(nB);
+            w.zoff = w.mem.alloc<long long>(B);
+            HIP_CHECK(hipMemsetAsync(w.zoff, 0, sizeof(long long) * B, op->s));
+        }
+    }
     w.Rhs = w.mem.alloc<cd>(nB);
     w.W = w.mem.alloc<cd>(nB);
     w.st = w.mem.alloc<SlotState>(B);
@@ -491,14 +543,15 @@ struct OpData<cf> {
 // y = A x for the batch: x element (row, b) at xin[in_off[b] + row*B + b];
 // grids, S* and FFT(G) in precision C, x and y complex128
 template <typename C>
-static void run_matvec(scadyn_operator *op, MatvecSpace &w, const cd *xin, const int *active, cd *y) {
+static void run_matvec(scadyn_operator *op, MatvecSpace &w, const C *xin, const int *active, cd *y,
+                       const long long *in_off = nullptr) {
     const HostGeometry &g = op->g;
     const int pr = OpData<C>::pr;
     C *g0 = (C *)w.grid0, *g1 = (C *)w.grid1;
     const C *tw = OpData<C>::tw(op);
     MatvecArgs<C> a;
     a.xin = xin;
-    a.in_off = w.in_off;
+    a.in_off = in_off ? in_off : w.in_off;
     a.B = w.B;
     a.N = g.N;
     a.Nx = g.Nx;
@@ -525,7 +578,7 @@ static void run_matvec(scadyn_operator *op, MatvecSpace &w, const cd *xin, const
     const double nB = 4.0 * w.B;
     const double Lx = op->L[0], Ly = op->L[1], Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
     {
-        ProfScope ps(op, SCADYN_KCLASS_SCATTER, c * (Nx * Ny * Nz * nB + 27.0 * 4 * g.N) + 16.0 * 3 * g.N * w.B);
+        ProfScope ps(op, SCADYN_KCLASS_SCATTER, c * (Nx * Ny * Nz * nB + 27.0 * 4 * g.N) + c * 3 * g.N * w.B);
         launch_scatter<C>(a, op->s);
     }
     {
@@ -558,7 +611,7 @@ static void run_matvec(scadyn_operator *op, MatvecSpace &w, const cd *xin, const
     }
     {
         ProfScope ps(op, SCADYN_KCLASS_GATHER,
-                     c * (Nx * Ny * Nz * nB + 27.0 * 4 * g.N) + 16.0 * 6 * g.N * w.B + 80.0 * op->nblocks);
+                     c * (Nx * Ny * Nz * nB + 27.0 * 4 * g.N) + (c + 16.0) * 3 * g.N * w.B + 80.0 * op->nblocks);
         launch_gather_near<C>(a, op->s);
     }
 }
@@ -688,11 +741,6 @@ static int auto_batch(scadyn_operator *op, int ncols) {
     return std::max(1, B);
 }
 
-// basis vectors per MGS pass when opts.mgs_block = 0: 4 moves 2.5 vectors of
-// HBM traffic per coefficient against 4 for one per pass, at 98 VGPRs (8 needs
-// 234 and runs slower at 2 waves per SIMD)
-constexpr int MGS_BLOCK_AUTO = 4;
-
 struct SolveSpec {
     int prec = SCADYN_PREC_FP64;  // matvec precision of the Arnoldi steps
     bool x0zero = false;          // x0 = 0 (correction equations) instead of x0 = rhs
@@ -763,6 +811,9 @@ struct ColumnSolver {
         spec = sp;
         B = auto_batch(op, (int)cols.size());
         w = &workspace(op, B);
+        // a complex64 basis serves only the mixed-precision blocked-MGS solve
+        if (w->Vf && (spec.prec != SCADYN_PREC_MIXED || spec.ortho != 0 || spec.mgs_block <= 1))
+            fail(SCADYN_ERR_ARG, "solve settings do not match the operator's complex64 Krylov basis");
         d.n = w->n;
         d.B = B;
         d.m = w->m;
@@ -778,7 +829,12 @@ struct ColumnSolver {
         slot_col.assign(B, -1);
         const size_t nB = (size_t)w->n * B;
         HIP_CHECK(hipMemsetAsync(w->st, 0, sizeof(SlotState) * B, op->s));
-        HIP_CHECK(hipMemsetAsync(w->V, 0, sizeof(cd) * (size_t)(w->m + 2) * nB, op->s));
+        if (w->Vf) {
+            HIP_CHECK(hipMemsetAsync(w->Vf, 0, sizeof(cf) * (size_t)(w->m + 1) * nB, op->s));
+            HIP_CHECK(hipMemsetAsync(w->X, 0, sizeof(cd) * nB, op->s));
+        } else {
+            HIP_CHECK(hipMemsetAsync(w->V, 0, sizeof(cd) * (size_t)(w->m + 2) * nB, op->s));
+        }
         // every system finishes within restart cycles of at most m+1 matvecs; bound the loop
         max_steps = ((long long)spec.restart * (w->m + 1) + 4) * (((long long)cols.size() + B - 1) / B + 2) + 16;
         HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
@@ -808,7 +864,7 @@ struct ColumnSolver {
         }
         if (any_load) {
             HIP_CHECK(hipMemcpyAsync(w->col, h_load.p, sizeof(int) * B, hipMemcpyHostToDevice, s));
-            launch_load_rhs(d, w->col, src, ld, w->Rhs, w->V, w->pnrm, spec.x0zero ? 1 : 0, s);
+            launch_load_rhs(d, w->col, src, ld, w->Rhs, w->X, w->pnrm, spec.x0zero ? 1 : 0, s);
             launch_load_finish(d, w->col, w->st, w->pnrm, s);
         }
         // MGS passes needed: max over busy slots of i+1 (host view of the previous step)
@@ -829,16 +885,20 @@ struct ColumnSolver {
             }
             launch_set_inputs(d, w->st, w->mv.in_off, w->mv.active, spec.verify64 ? 1 : 0, s);
             if (narn > 0) {
-                if (spec.prec == SCADYN_PREC_MIXED)
-                    run_matvec<cf>(op, w->mv, w->V, w->mv.active, w->W);
-                else
+                if (spec.prec == SCADYN_PREC_MIXED && w->Vf) {
+                    run_matvec<cf>(op, w->mv, w->Vf, w->mv.active, w->W);
+                } else if (spec.prec == SCADYN_PREC_MIXED) {
+                    launch_stage_f32(w->V, w->mv.in_off, w->mv.active, w->n, B, w->Xs, s);
+                    run_matvec<cf>(op, w->mv, w->Xs, w->mv.active, w->W, w->zoff);
+                } else {
                     run_matvec<cd>(op, w->mv, w->V, w->mv.active, w->W);
+                }
             }
             for (int r0 = 0; r0 < nrst; r0 += w->rs_cap) {
                 const int nr = std::min(w->rs_cap, nrst - r0);
                 for (int j = nr; j < w->rs_cap; ++j) h_rs.p[r0 + j] = -1;
                 HIP_CHECK(hipMemcpyAsync(w->rslot, h_rs.p + r0, sizeof(int) * w->rs_cap, hipMemcpyHostToDevice, s));
-                launch_pack_slots(d, w->V + (size_t)(w->m + 1) * w->n * B, w->rslot, w->rs_cap, w->rx, s);
+                launch_pack_slots(d, w->X, w->rslot, w->rs_cap, w->rx, s);
                 run_matvec<cd>(op, *w->rs, w->rx, nullptr, w->ry);
                 launch_unpack_slots(d, w->ry, w->rslot, w->rs_cap, w->W, s);
             }
@@ -849,9 +909,12 @@ struct ColumnSolver {
             } else if (spec.ortho == 2) {
                 launch_bmgs(d, w->W, w->V, w->Rhs, w->st, w->H, w->pdot, w->pnrm, w->pnav, w->cc, w->nrm2, w->nav2,
                             std::max(K - 1, 0), s);
+            } else if (spec.mgs_block > 1 && w->Vf) {
+                launch_mgsb(d, spec.mgs_block, K, w->W, (const cf *)w->Vf, w->Rhs, w->st, w->H, w->pdot, w->pnrm,
+                            w->pnav, w->nrm2, w->nav2, s);
             } else if (spec.mgs_block > 1) {
-                launch_mgsb(d, spec.mgs_block, K, w->W, w->V, w->Rhs, w->st, w->H, w->pdot, w->pnrm, w->pnav,
-                            w->nrm2, w->nav2, s);
+                launch_mgsb(d, spec.mgs_block, K, w->W, (const cd *)w->V, w->Rhs, w->st, w->H, w->pdot, w->pnrm,
+                            w->pnav, w->nrm2, w->nav2, s);
             } else {
                 for (int kk = 0; kk <= K; ++kk) {
                     launch_mgs_pass(d, kk, w->W, w->V, w->Rhs, w->st, w->H, w->pdot, w->pnrm, w->pnav, s);
@@ -860,7 +923,10 @@ struct ColumnSolver {
             }
             launch_slot_update(d, w->st, w->H, w->cs, w->sn, w->g, w->y, w->nrm2, w->nav2, spec.tol, spec.restart,
                                spec.verify64 ? 1 : 0, spec.tol_true, s);
-            launch_vec_update(d, w->st, w->W, w->V, w->y, s);
+            if (w->Vf)
+                launch_vec_update(d, w->st, w->W, w->Vf, w->X, w->y, s);
+            else
+                launch_vec_update(d, w->st, w->W, w->V, w->X, w->y, s);
         }
         HIP_CHECK(hipMemcpyAsync(hs.p, w->st, sizeof(SlotState) * B, hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipEventRecord(ev, s));
@@ -888,7 +954,7 @@ struct ColumnSolver {
         }
         if (any_done) {
             HIP_CHECK(hipMemcpyAsync(w->ret, h_ret.p, sizeof(int) * B, hipMemcpyHostToDevice, op->s));
-            launch_store_x(d, w->ret, w->V, dst, ld, op->s);
+            launch_store_x(d, w->ret, w->X, dst, ld, op->s);
             launch_retire(d, w->ret, w->st, op->s);
         }
     }
@@ -1365,10 +1431,14 @@ int scadyn_op_matvec_prec(scadyn_operator *op, int32_t precision, int32_t nrhs, 
     hipLaunchKernelGGL(k_pack, dim3(nblk(3LL * N * nrhs)), dim3(256), 0, s, hx, nrhs, N, op->d_perm, nrhs, xin);
     std::vector<long long> off(nrhs, 0);
     h2d_sync(w.mv.in_off, off.data(), sizeof(long long) * nrhs, s);
-    if (precision == SCADYN_PREC_MIXED)
-        run_matvec<cf>(op, w.mv, xin, nullptr, w.W);
-    else
+    if (precision == SCADYN_PREC_MIXED) {
+        cf *xf = tmp.alloc<cf>(nv);  // the complex64 matvec takes complex64 input
+        hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(256), 0, s, xin, xf, nv);
+        HIP_CHECK(hipGetLastError());
+        run_matvec<cf>(op, w.mv, xf, nullptr, w.W);
+    } else {
         run_matvec<cd>(op, w.mv, xin, nullptr, w.W);
+    }
     hipLaunchKernelGGL(k_unpack, dim3(nblk(3LL * N * nrhs)), dim3(256), 0, s, w.W, nrhs, N, op->d_perm, nrhs, hx);
     HIP_CHECK(hipMemcpyAsync(y, hx, sizeof(cd) * nv, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));

@@ -22,7 +22,8 @@
 // one cube's wave fetches is still in that XCD's L2 when the waves of the
 // cubes beside it along x ask for it.
 //
-// Buffers: grid [Nz][Ny][Nx][4][B], x and y [3N][B] (x at in_off[b]), S4
+// Buffers: grid [Nz][Ny][Nx][4][B], x [3N][B] in the matvec precision (at
+// in_off[b]) and y [3N][B] complex128, S4
 // [N][27][4], row_start [N+1], ent_blk [row_start[N]], blocks [P][9] complex64.
 #pragma once
 
@@ -36,7 +37,7 @@ namespace gcore {
 template <typename C>
 struct GatherArgs {
     const C *grid;
-    const cd *xin;
+    const C *xin;  // in the matvec precision
     const long long *in_off;
     int B;
     const int *ne_cubes;
@@ -170,7 +171,7 @@ GC_HD int64_t lane_row_start(const GatherArgs<C> &g, int tp, int nt, int lane) {
 template <typename C, int NT, int VC>
 GC_HD void chunk_part(const GatherArgs<C> &g, const Item &it, int v0, int nv, const C *sm, C (*acc)[3]) {
     const int B = g.B;
-    const cd *xb = g.xin + g.in_off[it.b] + it.b;
+    const C *xb = g.xin + g.in_off[it.b] + it.b;
 #pragma unroll 1
     for (int v = 0; v < nv; ++v) {
         const int col = v0 + v;

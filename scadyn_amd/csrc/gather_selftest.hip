@@ -119,8 +119,8 @@ static void run_case(int B, std::mt19937_64 &rng) {
     for (auto &z : S4) z = to<C>(cmk(nd(rng), nd(rng)));
     for (auto &z : grid) z = to<C>(cmk(nd(rng), nd(rng)));
     // x: slot b's vector at in_off[b] + row*B + b inside a buffer of two vector blocks
-    std::vector<cd> xin((size_t)2 * 3 * N * B);
-    for (auto &z : xin) z = cmk(nd(rng), nd(rng));
+    std::vector<C> xin((size_t)2 * 3 * N * B);  // in the matvec precision
+    for (auto &z : xin) z = to<C>(cmk(nd(rng), nd(rng)));
     std::vector<long long> in_off(B);
     std::vector<int> active(B);
     for (int b = 0; b < B; ++b) {
@@ -159,7 +159,7 @@ static void run_case(int B, std::mt19937_64 &rng) {
                     for (int i = 0; i < 3; ++i)
                         for (int jj = 0; jj < 3; ++jj) {
                             const cf m = (eb & 0x80000000u) ? a[jj + 3 * i] : a[i + 3 * jj];
-                            const cd xv = xin[in_off[b] + (size_t)(3 * v + jj) * B + b];
+                            const cd xv = Prec<cd>::from(xin[in_off[b] + (size_t)(3 * v + jj) * B + b]);
                             acc[i] = cadd(acc[i], cmul(Prec<cd>::from(m), xv));
                         }
                 }

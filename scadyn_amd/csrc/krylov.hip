@@ -147,9 +147,9 @@ __global__ __launch_bounds__(RED_T) void k_mgs_reduce(KrylovDims d, int k, const
 // ---- MGS evaluated BS basis vectors per pass (opts.ortho = 0, mgs_block > 1) ----
 // The per-thread row sweep and the per-slot coefficient step are in
 // mgsb_core.hpp (shared with the host self-test).
-template <int BS>
+template <int BS, typename VT>
 __global__ __launch_bounds__(256) void k_mgsb_pass(KrylovDims d, int p, cd *__restrict__ W,
-                                                   const cd *__restrict__ V, const cd *__restrict__ Rhs,
+                                                   const VT *__restrict__ V, const cd *__restrict__ Rhs,
                                                    const SlotState *__restrict__ st, const cd *__restrict__ H,
                                                    cd *__restrict__ pdot, double *__restrict__ pnrm,
                                                    double *__restrict__ pnav) {
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(256) void k_mgsb_pass(KrylovDims d, int p, cd *__re
     for (int q = 0; q < NQ; ++q) acc[q] = cmk(0.0, 0.0);
     double nrm = 0.0, nav = 0.0;
     if (m.ok)
-        mgsb::sweep_rows<BS>(d, p, W, V, Rhs, st, H, m.b, (long long)blockIdx.x * m.RW + m.rl,
+        mgsb::sweep_rows<BS, VT>(d, p, W, V, Rhs, st, H, m.b, (long long)blockIdx.x * m.RW + m.rl,
                              (long long)gridDim.x * m.RW, acc, nrm, nav);
     snrm[threadIdx.x] = nrm;
     snav[threadIdx.x] = nav;
@@ -210,28 +210,38 @@ __global__ __launch_bounds__(RED_T) void k_mgsb_reduce(KrylovDims d, int p, cons
     if (threadIdx.x == 0) mgsb::slot_coef<BS>(d, p, st, red, x1, x2, H, nrm2, nav2, b);
 }
 
-template <int BS>
-static void launch_mgsb_t(const KrylovDims &d, int K, cd *W, const cd *V, const cd *Rhs, const SlotState *st,
+template <int BS, typename VT>
+static void launch_mgsb_t(const KrylovDims &d, int K, cd *W, const VT *V, const cd *Rhs, const SlotState *st,
                           cd *H, cd *pdot, double *pnrm, double *pnav, double *nrm2, double *nav2, hipStream_t s) {
     const int P = (K + BS - 1) / BS;  // blocks of the vectors 0 .. K-1; pass P takes the last norm
     const int SB = d.B < 256 ? d.B : 256;
     const dim3 grid((unsigned)d.nwg, (unsigned)((d.B + SB - 1) / SB));
     for (int p = 0; p <= P; ++p) {
-        hipLaunchKernelGGL((k_mgsb_pass<BS>), grid, dim3(256), 0, s, d, p, W, V, Rhs, st, H, pdot, pnrm, pnav);
+        hipLaunchKernelGGL((k_mgsb_pass<BS, VT>), grid, dim3(256), 0, s, d, p, W, V, Rhs, st, H, pdot, pnrm, pnav);
         hipLaunchKernelGGL((k_mgsb_reduce<BS>), dim3((unsigned)d.B), dim3(RED_T), 0, s, d, p, st, H, pdot, pnrm,
                            pnav, nrm2, nav2);
     }
     HIP_CHECK(hipGetLastError());
 }
 
-void launch_mgsb(const KrylovDims &d, int bs, int K, cd *W, const cd *V, const cd *Rhs, const SlotState *st, cd *H,
-                 cd *pdot, double *pnrm, double *pnav, double *nrm2, double *nav2, hipStream_t s) {
+template <typename VT>
+static void launch_mgsb_v(const KrylovDims &d, int bs, int K, cd *W, const VT *V, const cd *Rhs,
+                          const SlotState *st, cd *H, cd *pdot, double *pnrm, double *pnav, double *nrm2,
+                          double *nav2, hipStream_t s) {
     switch (bs) {
-        case 2: launch_mgsb_t<2>(d, K, W, V, Rhs, st, H, pdot, pnrm, pnav, nrm2, nav2, s); break;
-        case 4: launch_mgsb_t<4>(d, K, W, V, Rhs, st, H, pdot, pnrm, pnav, nrm2, nav2, s); break;
-        case 8: launch_mgsb_t<8>(d, K, W, V, Rhs, st, H, pdot, pnrm, pnav, nrm2, nav2, s); break;
+        case 2: launch_mgsb_t<2, VT>(d, K, W, V, Rhs, st, H, pdot, pnrm, pnav, nrm2, nav2, s); break;
+        case 4: launch_mgsb_t<4, VT>(d, K, W, V, Rhs, st, H, pdot, pnrm, pnav, nrm2, nav2, s); break;
+        case 8: launch_mgsb_t<8, VT>(d, K, W, V, Rhs, st, H, pdot, pnrm, pnav, nrm2, nav2, s); break;
         default: fail(SCADYN_ERR_ARG, "MGS block size must be 2, 4 or 8");
     }
+}
+void launch_mgsb(const KrylovDims &d, int bs, int K, cd *W, const cd *V, const cd *Rhs, const SlotState *st, cd *H,
+                 cd *pdot, double *pnrm, double *pnav, double *nrm2, double *nav2, hipStream_t s) {
+    launch_mgsb_v<cd>(d, bs, K, W, V, Rhs, st, H, pdot, pnrm, pnav, nrm2, nav2, s);
+}
+void launch_mgsb(const KrylovDims &d, int bs, int K, cd *W, const cf *V, const cd *Rhs, const SlotState *st, cd *H,
+                 cd *pdot, double *pnrm, double *pnav, double *nrm2, double *nav2, hipStream_t s) {
+    launch_mgsb_v<cf>(d, bs, K, W, V, Rhs, st, H, pdot, pnrm, pnav, nrm2, nav2, s);
 }
 int mgsb_partials(int bs) { return bs + bs * (bs - 1) / 2; }
 
@@ -552,9 +562,11 @@ __global__ void k_slot_update(KrylovDims d, SlotState *__restrict__ st, cd *__re
     st[b] = s;
 }
 
+// new basis vector v = w / h (stored as VT), or at a cycle's end x += V y
+template <typename VT>
 __global__ __launch_bounds__(256) void k_vec_update(KrylovDims d, const SlotState *__restrict__ st,
-                                                    const cd *__restrict__ W, cd *__restrict__ V,
-                                                    const cd *__restrict__ yall) {
+                                                    const cd *__restrict__ W, VT *__restrict__ V,
+                                                    cd *__restrict__ X, const cd *__restrict__ yall) {
     const size_t nB = (size_t)d.n * d.B;
     const long long tot = d.n * d.B;
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
@@ -565,20 +577,19 @@ __global__ __launch_bounds__(256) void k_vec_update(KrylovDims d, const SlotStat
             const double h = (s.act == ACT_V0) ? s.beta : s.hnorm;
             cd w = W[e];
             if (h != 0.0) w = cmk(w.x / h, w.y / h);
-            V[(size_t)s.tgt * nB + e] = w;
+            V[(size_t)s.tgt * nB + e] = Prec<VT>::from(w);
         } else if (s.act == ACT_XUPD) {
             const cd *y = yall + (size_t)b * d.m;
             cd acc = cmk(0, 0);
-            for (int q = 0; q < s.len; ++q) cfma(acc, V[(size_t)q * nB + e], y[q]);
-            cd *x = V + (size_t)(d.m + 1) * nB + e;
-            *x = cadd(*x, acc);
+            for (int q = 0; q < s.len; ++q) cfma(acc, Prec<cd>::from(V[(size_t)q * nB + e]), y[q]);
+            X[e] = cadd(X[e], acc);
         }
     }
 }
 
 __global__ __launch_bounds__(256) void k_load_rhs(KrylovDims d, const int *__restrict__ col,
                                                   const cd *__restrict__ src, long long ld,
-                                                  cd *__restrict__ Rhs, cd *__restrict__ V,
+                                                  cd *__restrict__ Rhs, cd *__restrict__ X,
                                                   double *__restrict__ pnrm, int x0zero) {
     __shared__ double snrm[256];
     RowSlotMap m = row_slot_map(d.B);
@@ -591,7 +602,7 @@ __global__ __launch_bounds__(256) void k_load_rhs(KrylovDims d, const int *__res
             cd v = src[(size_t)row * ld + c];
             Rhs[e] = v;
             // x0 = rhs as the reference (src/gmres_module.f90:198), or 0 for correction equations
-            V[(size_t)(d.m + 1) * nB + e] = x0zero ? cmk(0, 0) : v;
+            X[e] = x0zero ? cmk(0, 0) : v;
             nrm += cabs2(v);
         }
     }
@@ -634,9 +645,8 @@ __global__ __launch_bounds__(RED_T) void k_load_finish(KrylovDims d, const int *
     st[b] = s;
 }
 
-__global__ void k_store_x(KrylovDims d, const int *__restrict__ col, const cd *__restrict__ V,
+__global__ void k_store_x(KrylovDims d, const int *__restrict__ col, const cd *__restrict__ X,
                           cd *__restrict__ dst, long long ld) {
-    const size_t nB = (size_t)d.n * d.B;
     const long long tot = d.n * d.B;
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
          e += (long long)gridDim.x * blockDim.x) {
@@ -644,7 +654,7 @@ __global__ void k_store_x(KrylovDims d, const int *__restrict__ col, const cd *_
         const int c = col[b];
         if (c < 0) continue;
         const long long row = e / d.B;
-        dst[(size_t)row * ld + c] = V[(size_t)(d.m + 1) * nB + e];
+        dst[(size_t)row * ld + c] = X[e];
     }
 }
 
@@ -791,16 +801,25 @@ void launch_slot_update(const KrylovDims &d, SlotState *st, cd *H, cd *cs, cd *s
                        tol, restart, verify, tol_true);
     HIP_CHECK(hipGetLastError());
 }
-void launch_vec_update(const KrylovDims &d, const SlotState *st, const cd *W, cd *V, const cd *y,
-                       hipStream_t s) {
+template <typename VT>
+static void launch_vec_update_t(const KrylovDims &d, const SlotState *st, const cd *W, VT *V, cd *X, const cd *y,
+                                hipStream_t s) {
     long long tot = d.n * d.B;
     unsigned g = (unsigned)std::min<long long>((tot + 255) / 256, 1 << 16);
-    hipLaunchKernelGGL(k_vec_update, dim3(g), dim3(256), 0, s, d, st, W, V, y);
+    hipLaunchKernelGGL((k_vec_update<VT>), dim3(g), dim3(256), 0, s, d, st, W, V, X, y);
     HIP_CHECK(hipGetLastError());
 }
-void launch_load_rhs(const KrylovDims &d, const int *col, const cd *src, long long ld, cd *Rhs, cd *V,
+void launch_vec_update(const KrylovDims &d, const SlotState *st, const cd *W, cd *V, cd *X, const cd *y,
+                       hipStream_t s) {
+    launch_vec_update_t<cd>(d, st, W, V, X, y, s);
+}
+void launch_vec_update(const KrylovDims &d, const SlotState *st, const cd *W, cf *V, cd *X, const cd *y,
+                       hipStream_t s) {
+    launch_vec_update_t<cf>(d, st, W, V, X, y, s);
+}
+void launch_load_rhs(const KrylovDims &d, const int *col, const cd *src, long long ld, cd *Rhs, cd *X,
                      double *pnrm, int x0zero, hipStream_t s) {
-    hipLaunchKernelGGL(k_load_rhs, rs_grid(d), dim3(256), 0, s, d, col, src, ld, Rhs, V, pnrm, x0zero);
+    hipLaunchKernelGGL(k_load_rhs, rs_grid(d), dim3(256), 0, s, d, col, src, ld, Rhs, X, pnrm, x0zero);
     HIP_CHECK(hipGetLastError());
 }
 void launch_load_finish(const KrylovDims &d, const int *col, SlotState *st, const double *pnrm,
@@ -808,10 +827,10 @@ void launch_load_finish(const KrylovDims &d, const int *col, SlotState *st, cons
     hipLaunchKernelGGL(k_load_finish, dim3((unsigned)d.B), dim3(RED_T), 0, s, d, col, st, pnrm);
     HIP_CHECK(hipGetLastError());
 }
-void launch_store_x(const KrylovDims &d, const int *col, const cd *V, cd *dst, long long ld, hipStream_t s) {
+void launch_store_x(const KrylovDims &d, const int *col, const cd *X, cd *dst, long long ld, hipStream_t s) {
     long long tot = d.n * d.B;
     unsigned g = (unsigned)std::min<long long>((tot + 255) / 256, 1 << 16);
-    hipLaunchKernelGGL(k_store_x, dim3(g), dim3(256), 0, s, d, col, V, dst, ld);
+    hipLaunchKernelGGL(k_store_x, dim3(g), dim3(256), 0, s, d, col, X, dst, ld);
     HIP_CHECK(hipGetLastError());
 }
 void launch_bmgs(const KrylovDims &d, cd *W, const cd *V, const cd *Rhs, const SlotState *st, cd *H, cd *pq,

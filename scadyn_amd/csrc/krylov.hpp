@@ -45,15 +45,18 @@ void launch_mgs_reduce(const KrylovDims &d, int k, const SlotState *st, cd *H, c
 void launch_slot_update(const KrylovDims &d, SlotState *st, cd *H, cd *cs, cd *sn, cd *g, cd *y,
                         const double *nrm2, const double *nav2, double tol, int restart, int verify, double tol_true,
                         hipStream_t s);
-void launch_vec_update(const KrylovDims &d, const SlotState *st, const cd *W, cd *V, const cd *y,
+// basis V stored as complex128 or (mixed precision) complex64; x in X (n x B)
+void launch_vec_update(const KrylovDims &d, const SlotState *st, const cd *W, cd *V, cd *X, const cd *y,
+                       hipStream_t s);
+void launch_vec_update(const KrylovDims &d, const SlotState *st, const cd *W, cf *V, cd *X, const cd *y,
                        hipStream_t s);
 // load column col[b] (>= 0) of src (n x ld, row-major) into Rhs and X of slot b
-void launch_load_rhs(const KrylovDims &d, const int *col, const cd *src, long long ld, cd *Rhs, cd *V,
+void launch_load_rhs(const KrylovDims &d, const int *col, const cd *src, long long ld, cd *Rhs, cd *X,
                      double *pnrm, int x0zero, hipStream_t s);
 void launch_load_finish(const KrylovDims &d, const int *col, SlotState *st, const double *pnrm,
                         hipStream_t s);
 // store X of slots with col[b] >= 0 into column col[b] of dst (n x ld)
-void launch_store_x(const KrylovDims &d, const int *col, const cd *V, cd *dst, long long ld, hipStream_t s);
+void launch_store_x(const KrylovDims &d, const int *col, const cd *X, cd *dst, long long ld, hipStream_t s);
 
 // classical Gram-Schmidt + DGKS for column i of every Arnoldi slot (and the
 // restart residual); kmax = max over slots of i.  pdot holds nwg*(m+1)*B.
@@ -72,6 +75,8 @@ int bmgs_partials();
 // rows (mgsb_core.hpp): passes 0 .. ceil(K/bs) with K = max over Arnoldi slots
 // of i+1 (0 when only restarting slots).  pdot holds nwg*mgsb_partials(bs)*B.
 void launch_mgsb(const KrylovDims &d, int bs, int K, cd *W, const cd *V, const cd *Rhs, const SlotState *st, cd *H,
+                 cd *pdot, double *pnrm, double *pnav, double *nrm2, double *nav2, hipStream_t s);
+void launch_mgsb(const KrylovDims &d, int bs, int K, cd *W, const cf *V, const cd *Rhs, const SlotState *st, cd *H,
                  cd *pdot, double *pnrm, double *pnav, double *nrm2, double *nav2, hipStream_t s);
 int mgsb_partials(int bs);
 

@@ -14,7 +14,7 @@ namespace scadyn {
 // scatter (sparse_T_matmul, src/sparse.f90:13-43, + vec2arr2), node-centric:
 // node p sums the stencil weights of the tets of the <= 8 cubes whose box holds p.
 template <typename C>
-__global__ __launch_bounds__(256) void k_scatter(const cd *__restrict__ xin,
+__global__ __launch_bounds__(256) void k_scatter(const C *__restrict__ xin,
                                                  const long long *__restrict__ in_off, int B,
                                                  const int *__restrict__ cube_start,
                                                  const C *__restrict__ S4, int Nx, int Ny, int Nz,
@@ -25,7 +25,7 @@ __global__ __launch_bounds__(256) void k_scatter(const cd *__restrict__ xin,
         const int b = (int)(e % B);
         const long long p = e / B;
         const int px = (int)(p % Nx), py = (int)((p / Nx) % Ny), pz = (int)(p / ((long long)Nx * Ny));
-        const cd *xb = xin + in_off[b] + b;
+        const C *xb = xin + in_off[b] + b;
         C a0 = Prec<C>::zero(), a1 = a0, a2 = a0, a3 = a0;
         // cubes c with 2c <= p <= 2c+2 per axis
         const int cx0 = max(0, (px - 1) / 2), cx1 = min(ncx - 1, px / 2);
@@ -62,7 +62,7 @@ __global__ __launch_bounds__(256) void k_scatter(const cd *__restrict__ xin,
 // + symmetric near-zone blocks (matmul_Acorr_const, src/sparse_mat.f90:96-123).
 template <typename C>
 __global__ __launch_bounds__(256) void k_gather_near(
-    const C *__restrict__ grid, const cd *__restrict__ xin, const long long *__restrict__ in_off, int B,
+    const C *__restrict__ grid, const C *__restrict__ xin, const long long *__restrict__ in_off, int B,
     int N, const int *__restrict__ tet_cube, const C *__restrict__ S4, int Nx, int Ny, int ncx, int ncy,
     const int64_t *__restrict__ row_start, const int *__restrict__ ent_col,
     const uint32_t *__restrict__ ent_blk, const cf *__restrict__ blocks, double k2,
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) void k_gather_near(
                     cfma(Z2, sj[3], g3);
                 }
         // near zone
-        const cd *xb = xin + in_off[b] + b;
+        const C *xb = xin + in_off[b] + b;
         C n0 = Prec<C>::zero(), n1 = n0, n2 = n0;
         for (int64_t q = row_start[t]; q < row_start[t + 1]; ++q) {
             const int col = ent_col[q];
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(256) void k_gather_near(
 // load and only the slot-indexed x, grid and y accesses are vector loads,
 // each 64 consecutive complex numbers.
 template <typename C>
-__global__ __launch_bounds__(256) void k_scatter_w64(const cd *__restrict__ xin,
+__global__ __launch_bounds__(256) void k_scatter_w64(const C *__restrict__ xin,
                                                      const long long *__restrict__ in_off, int B,
                                                      const int *__restrict__ cube_start,
                                                      const C *__restrict__ S4, int Nx, int Ny, int Nz,
@@ -148,7 +148,7 @@ __global__ __launch_bounds__(256) void k_scatter_w64(const cd *__restrict__ xin,
         const bool valid = braw < B;
         const int b = valid ? braw : B - 1;
         const int px = (int)(p % Nx), py = (int)((p / Nx) % Ny), pz = (int)(p / ((long long)Nx * Ny));
-        const cd *xb = xin + in_off[b] + b;
+        const C *xb = xin + in_off[b] + b;
         C a0 = Prec<C>::zero(), a1 = a0, a2 = a0, a3 = a0;
         const int cx0 = max(0, (px - 1) / 2), cx1 = min(ncx - 1, px / 2);
         const int cy0 = max(0, (py - 1) / 2), cy1 = min(ncy - 1, py / 2);
@@ -185,7 +185,7 @@ __global__ __launch_bounds__(256) void k_scatter_w64(const cd *__restrict__ xin,
 
 template <typename C>
 __global__ __launch_bounds__(256) void k_gather_near_w64(
-    const C *__restrict__ grid, const cd *__restrict__ xin, const long long *__restrict__ in_off, int B,
+    const C *__restrict__ grid, const C *__restrict__ xin, const long long *__restrict__ in_off, int B,
     int N, const int *__restrict__ tet_cube, const C *__restrict__ S4, int Nx, int Ny, int ncx, int ncy,
     const int64_t *__restrict__ row_start, const int *__restrict__ ent_col,
     const uint32_t *__restrict__ ent_blk, const cf *__restrict__ blocks, double k2,
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(256) void k_gather_near_w64(
                     cfma(Z1, sj[2], g3);
                     cfma(Z2, sj[3], g3);
                 }
-        const cd *xb = xin + in_off[b] + b;
+        const C *xb = xin + in_off[b] + b;
         C n0 = Prec<C>::zero(), n1 = n0, n2 = n0;
         const int64_t q1 = row_start[t + 1];
         for (int64_t q = row_start[t]; q < q1; ++q) {
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(256) void k_gather_near_w64(
 // of once per node.  Per node the sum runs over cubes and tets in the same
 // order as k_scatter.
 template <typename C>
-__global__ __launch_bounds__(256) void k_scatter_blk_w64(const cd *__restrict__ xin,
+__global__ __launch_bounds__(256) void k_scatter_blk_w64(const C *__restrict__ xin,
                                                          const long long *__restrict__ in_off, int B,
                                                          const int *__restrict__ cube_start,
                                                          const C *__restrict__ S4, int Nx, int Ny, int Nz,
@@ -271,20 +271,17 @@ __global__ __launch_bounds__(256) void k_scatter_blk_w64(const cd *__restrict__ 
     const int nch = (B + 63) >> 6;  // last chunk may be partial: lanes >= B load slot B-1, store nothing
     // ceil(N/2) blocks per axis cover every node (nodes beyond the cubes' boxes get zeros)
     const int nbx = (Nx + 1) >> 1, nby = (Ny + 1) >> 1, nbz = (Nz + 1) >> 1;
-    const long long nblk = (long long)nbx * nby * nbz, nw = nblk * nch;
+    const long long nw = (long long)nbx * nby * nbz * nch;
     const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    // items in chunk-major order (one slot chunk sweeps all node blocks), each
-    // XCD a contiguous range of workgroups when the grid covers all items, so
-    // that node blocks adjacent along x share their tets' x in one L2
-    const long long G = gridDim.x;
-    const long long wg = G * 4 >= nw ? gcore::xcd_remap(blockIdx.x, G) : blockIdx.x;
-    for (long long w = wg * 4 + wib; w < nw; w += G * 4) {
-        const long long blk = w % nblk;
-        const int braw = (int)(w / nblk) * 64 + lane;
+    // items (node block, slot chunk) with the chunk fastest; a chunk-major order
+    // with XCD-contiguous workgroup ranges (as in the gather) measured slower
+    for (long long w = (long long)blockIdx.x * 4 + wib; w < nw; w += (long long)gridDim.x * 4) {
+        const long long blk = w / nch;
+        const int braw = (int)(w - blk * nch) * 64 + lane;
         const bool valid = braw < B;
         const int b = valid ? braw : B - 1;
         const int ba = (int)(blk % nbx), bb = (int)((blk / nbx) % nby), bc = (int)(blk / ((long long)nbx * nby));
-        const cd *xb = xin + in_off[b] + b;
+        const C *xb = xin + in_off[b] + b;
         C acc[2][2][2][4];
 #pragma unroll
         for (int iz = 0; iz < 2; ++iz)
@@ -364,7 +361,7 @@ __global__ __launch_bounds__(256) void k_scatter_blk_w64(const cd *__restrict__ 
 // GT tets per pass; y_c = near_c + Sc'^T g3 - k^2 S^T g_c accumulated directly.
 template <typename C, int GT>
 __global__ __launch_bounds__(256) void k_gather_cube_w64(
-    const C *__restrict__ grid, const cd *__restrict__ xin, const long long *__restrict__ in_off, int B,
+    const C *__restrict__ grid, const C *__restrict__ xin, const long long *__restrict__ in_off, int B,
     const int *__restrict__ ne_cubes, int n_ne, const int *__restrict__ cube_start, const C *__restrict__ S4,
     int Nx, int Ny, int ncx, int ncy, int ncz, const int64_t *__restrict__ row_start,
     const uint32_t *__restrict__ ent_blk, const cf *__restrict__ blocks, double k2,
@@ -384,7 +381,7 @@ __global__ __launch_bounds__(256) void k_gather_cube_w64(
         const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
         const int t0 = cube_start[c], t1 = cube_start[c + 1];
         const bool act = slot_active ? slot_active[b] != 0 : true;
-        const cd *xb = xin + in_off[b] + b;
+        const C *xb = xin + in_off[b] + b;
         const size_t base = (size_t)(2 * cx) + (size_t)Nx * ((size_t)(2 * cy) + (size_t)Ny * (2 * cz));
         for (int tp = t0; tp < t1; tp += GT) {
             const int nt = min(GT, t1 - tp);

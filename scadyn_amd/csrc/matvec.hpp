@@ -7,8 +7,9 @@ namespace scadyn {
 
 template <typename C>
 struct MatvecArgs {
-    // input multi-vector: element (row, b) at xin[in_off[b] + row*B + b]
-    const cd *xin = nullptr;
+    // input multi-vector in the matvec precision: element (row, b) at
+    // xin[in_off[b] + row*B + b]
+    const C *xin = nullptr;
     const long long *in_off = nullptr;
     int B = 0;
     int N = 0;

@@ -17,7 +17,7 @@
 // rhs - A x and its |.|^2.
 //
 // Buffers (m = Krylov dimension, n rows, B slots, nwg row-sweep workgroups):
-//   V (m+2) x n x B    W, Rhs n x B    H B x (m+1) x m
+//   V >= (m+1) x n x B (complex128 or complex64)    W, Rhs n x B    H B x (m+1) x m
 //   partials: dots/Gram nwg x NQ x B, norms nwg x B each
 #pragma once
 
@@ -59,9 +59,11 @@ MGSB_HD PassRole pass_role(const SlotState &s, int p) {
 }
 
 // one thread's rows (row0, row0 + rstep, ...) of slot b: partial dots/Gram in
-// acc[NQ], |w|^2 after the pass's update in nrm, before orthogonalisation in nav
-template <int BS>
-MGSB_HD void sweep_rows(const KrylovDims &d, int p, cd *W, const cd *V, const cd *Rhs, const SlotState *st,
+// acc[NQ], |w|^2 after the pass's update in nrm, before orthogonalisation in nav.
+// The basis is stored as VT (complex128, or complex64 for the mixed-precision
+// solve); all arithmetic is FP64.
+template <int BS, typename VT>
+MGSB_HD void sweep_rows(const KrylovDims &d, int p, cd *W, const VT *V, const cd *Rhs, const SlotState *st,
                         const cd *H, int b, long long row0, long long rstep, cd *acc, double &nrm, double &nav) {
     constexpr int NQ = Blk<BS>::NQ;
 #pragma unroll
@@ -88,13 +90,14 @@ MGSB_HD void sweep_rows(const KrylovDims &d, int p, cd *W, const cd *V, const cd
         if (r.nu > 0) {
 #pragma unroll
             for (int u = 0; u < BS; ++u)
-                if (u < r.nu) w = csub(w, cmul(h[u], V[(size_t)(r.ulo + u) * nB + e]));
+                if (u < r.nu) w = csub(w, cmul(h[u], Prec<cd>::from(V[(size_t)(r.ulo + u) * nB + e])));
             W[e] = w;
         }
         if (r.nd > 0) {
             cd v[BS];
 #pragma unroll
-            for (int u = 0; u < BS; ++u) v[u] = u < r.nd ? V[(size_t)(r.dlo + u) * nB + e] : cmk(0.0, 0.0);
+            for (int u = 0; u < BS; ++u)
+                v[u] = u < r.nd ? Prec<cd>::from(V[(size_t)(r.dlo + u) * nB + e]) : cmk(0.0, 0.0);
 #pragma unroll
             for (int j = 0; j < BS; ++j) {
                 acc[j] = cadd(acc[j], cmulc(v[j], w));

@@ -9,10 +9,14 @@
 // are set up in every phase (Arnoldi at columns 0, BS-1, BS, 2BS-1, 2BS, m-1;
 // restarting; idle; done) and the resulting H column, |w|^2 before and after,
 // and the orthogonalised w are compared with plain one-vector-at-a-time MGS.
-// Block sizes 2, 4, 8; batch widths 5, 64, 112, 256, 300; Krylov dimensions 20
-// and 50; the default number of row-sweep workgroups and 7.
+// Block sizes 2, 4, 8; basis stored as complex128 and complex64 (FP64
+// arithmetic on the stored values either way); batch widths 5, 64, 112, 256,
+// 300; Krylov dimensions 20 and 50; the default number of row-sweep workgroups
+// and 7.
 //
-// Exit status 0 when every case agrees to 1e-12 relative.
+// Exit status 0 when every case agrees to 1e-12 relative (H entries relative
+// to |v_k| |w|, the scale of the sum each is taken from: with a random,
+// non-orthogonal basis a coefficient can be far smaller than its terms).
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -28,7 +32,7 @@ static int failures = 0;
 
 static double crel(cd a, cd b, double scale) { return std::sqrt(cabs2(csub(a, b))) / scale; }
 
-template <int BS>
+template <int BS, typename VT>
 static void run_case(int B, int m, long long n, int nwg, std::mt19937_64 &rng) {
     constexpr int NQ = Blk<BS>::NQ;
     KrylovDims d;
@@ -39,8 +43,21 @@ static void run_case(int B, int m, long long n, int nwg, std::mt19937_64 &rng) {
     const size_t nB = (size_t)n * B;
     std::normal_distribution<double> nd;
     auto rnd = [&]() { return cmk(nd(rng), nd(rng)); };
-    std::vector<cd> V((size_t)(m + 2) * nB), W(nB), Rhs(nB), H((size_t)B * (m + 1) * m);
-    for (auto &z : V) z = rnd();
+    std::vector<VT> V((size_t)(m + 1) * nB);  // the basis as stored (complex128 or complex64)
+    std::vector<cd> W(nB), Rhs(nB), H((size_t)B * (m + 1) * m);
+    for (auto &z : V) z = Prec<VT>::from(rnd());
+    // unit basis vectors per slot (not orthogonal): each MGS step is then a
+    // projection and |w| can only shrink, so errors are bounded by |w0|
+    for (int k = 0; k <= m; ++k)
+        for (int b = 0; b < B; ++b) {
+            double a = 0.0;
+            for (long long r = 0; r < n; ++r) a += cabs2(Prec<cd>::from(V[(size_t)k * nB + (size_t)r * B + b]));
+            const double f = 1.0 / std::sqrt(a);
+            for (long long r = 0; r < n; ++r) {
+                VT &z = V[(size_t)k * nB + (size_t)r * B + b];
+                z = Prec<VT>::from(cscale(Prec<cd>::from(z), f));
+            }
+        }
     for (auto &z : W) z = rnd();
     for (auto &z : Rhs) z = rnd();
     for (auto &z : H) z = rnd();
@@ -82,7 +99,7 @@ static void run_case(int B, int m, long long n, int nwg, std::mt19937_64 &rng) {
                     for (int q = 0; q < NQ; ++q) acc[(size_t)t * NQ + q] = cmk(0.0, 0.0);
                     snrm[t] = snav[t] = 0.0;
                     if (ok)
-                        sweep_rows<BS>(d, p, W.data(), V.data(), Rhs.data(), st.data(), H.data(), b,
+                        sweep_rows<BS, VT>(d, p, W.data(), V.data(), Rhs.data(), st.data(), H.data(), b,
                                        (long long)bx * RW + rl, (long long)nwg * RW, &acc[(size_t)t * NQ], snrm[t],
                                        snav[t]);
                 }
@@ -129,19 +146,27 @@ static void run_case(int B, int m, long long n, int nwg, std::mt19937_64 &rng) {
             for (auto &z : w) w0 += cabs2(z);
             for (int k = 0; k <= s.i; ++k) {
                 cd h = cmk(0.0, 0.0);
-                for (long long r = 0; r < n; ++r) h = cadd(h, cmulc(V[(size_t)k * nB + (size_t)r * B + b], w[r]));
-                for (long long r = 0; r < n; ++r) w[r] = csub(w[r], cmul(h, V[(size_t)k * nB + (size_t)r * B + b]));
+                auto vk = [&](long long r) { return Prec<cd>::from(V[(size_t)k * nB + (size_t)r * B + b]); };
+                double vn = 0.0, wn0 = 0.0;  // scale of the sum: |v_k| |w| (h itself may cancel)
+                for (long long r = 0; r < n; ++r) {
+                    vn += cabs2(vk(r));
+                    wn0 += cabs2(w[r]);
+                }
+                for (long long r = 0; r < n; ++r) h = cadd(h, cmulc(vk(r), w[r]));
+                for (long long r = 0; r < n; ++r) w[r] = csub(w[r], cmul(h, vk(r)));
                 const cd got = H[(size_t)b * (m + 1) * m + (size_t)(m + 1) * s.i + k];
-                worst = std::max(worst, crel(got, h, std::sqrt(cabs2(h)) + 1e-300));
+                worst = std::max(worst, crel(got, h, std::sqrt(vn * wn0) + 1e-300));
             }
             double wn = 0.0, dw = 0.0;
             for (long long r = 0; r < n; ++r) {
                 wn += cabs2(w[r]);
                 dw = std::max(dw, std::sqrt(cabs2(csub(w[r], W[(size_t)r * B + b]))));
             }
-            worst = std::max(worst, std::fabs(nrm2[b] - wn) / wn);
+            // relative to the norm before orthogonalisation (the scale of the
+            // rounding; w itself can shrink by orders of magnitude)
+            worst = std::max(worst, std::fabs(nrm2[b] - wn) / w0);
             worst = std::max(worst, std::fabs(nav2[b] - w0) / w0);
-            worst = std::max(worst, dw / std::sqrt(wn / n));
+            worst = std::max(worst, dw / std::sqrt(w0 / n));
         } else if (s.phase == PH_RESTART) {
             double wn = 0.0, dw = 0.0;
             for (long long r = 0; r < n; ++r) {
@@ -161,8 +186,8 @@ static void run_case(int B, int m, long long n, int nwg, std::mt19937_64 &rng) {
     }
     const bool ok = worst < 1e-12;
     if (!ok) ++failures;
-    std::printf("BS=%d B=%3d m=%2d n=%5lld nwg=%4d passes=%2d  worst rel.err %.2e %s\n", BS, B, m, n, nwg, P + 1,
-                worst, ok ? "ok" : "FAIL");
+    std::printf("BS=%d %s B=%3d m=%2d n=%5lld nwg=%4d passes=%2d  worst rel.err %.2e %s\n", BS,
+                sizeof(VT) == 16 ? "c128" : "c64 ", B, m, n, nwg, P + 1, worst, ok ? "ok" : "FAIL");
 }
 
 int main() {
@@ -174,10 +199,12 @@ int main() {
             const long long n = 600;
             const int rw = 256 / std::min(B, 256);
             const int nwg = (int)std::max(1LL, std::min(2048LL, (n + rw * 16 - 1) / (rw * 16)));
-            run_case<2>(B, m, n, nwg, rng);
-            run_case<4>(B, m, n, nwg, rng);
-            run_case<8>(B, m, n, nwg, rng);
-            run_case<4>(B, m, n, 7, rng);
+            run_case<2, cd>(B, m, n, nwg, rng);
+            run_case<4, cd>(B, m, n, nwg, rng);
+            run_case<8, cd>(B, m, n, nwg, rng);
+            run_case<4, cd>(B, m, n, 7, rng);
+            run_case<4, cf>(B, m, n, nwg, rng);  // complex64 basis of the mixed-precision solve
+            run_case<8, cf>(B, m, n, 7, rng);
         }
     std::printf("%d failures\n", failures);
     return failures ? 1 : 0;
