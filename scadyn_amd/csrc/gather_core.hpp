@@ -9,9 +9,10 @@
 //    src/gmres_module.f90:148-153);
 //  * near part (matmul_Acorr_const, src/sparse_mat.f90:96-123): for each near
 //    cube, in chunks of VC column tets, lane (u, v) = (lane / VC, lane % VC)
-//    fetches the block of row tet u and column tet v, converts it once and
-//    stores it as a dense 3x3 (transposed when the pair is stored as (v, u));
-//    every lane then applies the chunk's blocks to its slot's x of those tets.
+//    fetches the block of row tet u and column tet v and stores it (complex64,
+//    as stored) as a dense 3x3, transposed when the pair is stored as (v, u);
+//    every lane then widens the chunk's blocks to the matvec precision and
+//    applies them to its slot's x of those tets.
 // The rows of a cube's tets list the near tets in the same order (near cube
 // offset, then tet id), so entry q of every row of the cube is column q.
 //
@@ -58,9 +59,13 @@ struct GatherArgs {
 // staged near chunk (GT x VC = 64: one block per lane)
 constexpr int GATHER_GT = 8, GATHER_VC = 8;
 
-template <int GT, int VC>
+// staging buffer: the S* records of GT tets (precision C) or, later in the
+// pass, GT x VC near-zone blocks (complex64, as stored)
+template <typename C, int GT, int VC>
 struct Shape {
-    static constexpr int LDS_N = (GT * 108 > GT * VC * 9) ? GT * 108 : GT * VC * 9;  // staging elements
+    static constexpr int S4_BYTES = GT * 108 * (int)sizeof(C);
+    static constexpr int BLK_BYTES = GT * VC * 9 * (int)sizeof(cf);
+    static constexpr int BYTES = S4_BYTES > BLK_BYTES ? S4_BYTES : BLK_BYTES;
 };
 
 // Workgroup b of a G-workgroup launch -> item index, a bijection of [0, G).
@@ -120,44 +125,65 @@ GC_HD void grid_part(const GatherArgs<C> &g, const Item &it, const C *sm, C (*ac
     typedef typename Prec<C>::real R;
     const R kk = (R)g.k2;
     const int B = g.B;
-#pragma unroll 1
-    for (int jz = 0; jz < 3; ++jz)
-#pragma unroll 1
-        for (int jy = 0; jy < 3; ++jy)
-#pragma unroll 1
-            for (int jx = 0; jx < 3; ++jx) {
-                const int j = jx + 3 * jy + 9 * jz;
-                const C *gp = g.grid + (it.base + jx + (size_t)g.Nx * (jy + (size_t)g.Ny * jz)) * 4 * B + it.b;
-                const C g0 = gp[0], g1 = gp[(size_t)B], g2 = gp[(size_t)2 * B], g3 = gp[(size_t)3 * B];
+    // box node j = jx + 3 jy + 9 jz.  In complex64 the 4 channels of the next
+    // node are fetched while the current node is applied; complex128 has no
+    // registers to spare for that at 3 waves per SIMD.
+    constexpr bool PREFETCH = sizeof(C) == 8;
+    auto node = [&](int j) {
+        const int jx = j % 3, jy = (j / 3) % 3, jz = j / 9;
+        return g.grid + (it.base + jx + (size_t)g.Nx * (jy + (size_t)g.Ny * jz)) * 4 * B + it.b;
+    };
+    C nx[4];
+    if (PREFETCH) {
+        const C *gp = node(0);
 #pragma unroll
-                for (int u = 0; u < NT; ++u) {
-                    const C *s = sm + u * 108 + j * 4;
-                    const C s0 = cscale(s[0], -kk);
-                    cfma(acc[u][0], s0, g0);
-                    cfma(acc[u][1], s0, g1);
-                    cfma(acc[u][2], s0, g2);
-                    cfma(acc[u][0], s[1], g3);
-                    cfma(acc[u][1], s[2], g3);
-                    cfma(acc[u][2], s[3], g3);
-                }
+        for (int c = 0; c < 4; ++c) nx[c] = gp[(size_t)c * B];
+    }
+#pragma unroll 1
+    for (int j = 0; j < 27; ++j) {
+        C g0, g1, g2, g3;
+        if (PREFETCH) {
+            g0 = nx[0], g1 = nx[1], g2 = nx[2], g3 = nx[3];
+            if (j + 1 < 27) {
+                const C *gp = node(j + 1);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) nx[c] = gp[(size_t)c * B];
             }
+        } else {
+            const C *gp = node(j);
+            g0 = gp[0], g1 = gp[(size_t)B], g2 = gp[(size_t)2 * B], g3 = gp[(size_t)3 * B];
+        }
+#pragma unroll
+        for (int u = 0; u < NT; ++u) {
+            const C *s = sm + u * 108 + j * 4;
+            const C s0 = cscale(s[0], -kk);
+            cfma(acc[u][0], s0, g0);
+            cfma(acc[u][1], s0, g1);
+            cfma(acc[u][2], s0, g2);
+            cfma(acc[u][0], s[1], g3);
+            cfma(acc[u][1], s[2], g3);
+            cfma(acc[u][2], s[3], g3);
+        }
+    }
 }
 
 // lane's block of a near chunk: row tet tp + lane / VC, column v0 + lane % VC,
-// entry q + lane % VC of the row (rsu = row_start of the lane's row tet)
+// entry q + lane % VC of the row (rsu = row_start of the lane's row tet).  The
+// block stays complex64 (its stored precision) in LDS and is widened when used:
+// half the LDS bytes of a complex128 copy for the FP64 matvec, same values.
 template <typename C, int GT, int VC>
-GC_HD void stage_blocks(const GatherArgs<C> &g, int nt, int nv, int q, int lane, int64_t rsu, C *sm) {
+GC_HD void stage_blocks(const GatherArgs<C> &g, int nt, int nv, int q, int lane, int64_t rsu, cf *sm) {
     static_assert(GT * VC <= 64, "one block per lane");
     const int su = lane / VC, sv = lane % VC;
     if (su >= nt || sv >= nv) return;
     const uint32_t eb = g.ent_blk[rsu + q + sv];
     const cf *a = g.blocks + (size_t)(eb & 0x7fffffffu) * 9;
-    C *dst = sm + (su * VC + sv) * 9;
+    cf *dst = sm + (su * VC + sv) * 9;
     if (eb & 0x80000000u) {
         for (int i = 0; i < 3; ++i)
-            for (int jj = 0; jj < 3; ++jj) dst[i + 3 * jj] = Prec<C>::from(a[jj + 3 * i]);
+            for (int jj = 0; jj < 3; ++jj) dst[i + 3 * jj] = a[jj + 3 * i];
     } else {
-        for (int i = 0; i < 9; ++i) dst[i] = Prec<C>::from(a[i]);
+        for (int i = 0; i < 9; ++i) dst[i] = a[i];
     }
 }
 
@@ -169,26 +195,29 @@ GC_HD int64_t lane_row_start(const GatherArgs<C> &g, int tp, int nt, int lane) {
 
 // apply the staged blocks of column tets v0 .. v0+nv-1
 template <typename C, int NT, int VC>
-GC_HD void chunk_part(const GatherArgs<C> &g, const Item &it, int v0, int nv, const C *sm, C (*acc)[3]) {
+GC_HD void chunk_part(const GatherArgs<C> &g, const Item &it, int v0, int nv, const cf *sm, C (*acc)[3]) {
     const int B = g.B;
     const C *xb = g.xin + g.in_off[it.b] + it.b;
+    // x of the next column tet is fetched while the current one is applied
+    C n0 = xb[(size_t)(3 * v0) * B], n1 = xb[(size_t)(3 * v0 + 1) * B], n2 = xb[(size_t)(3 * v0 + 2) * B];
 #pragma unroll 1
     for (int v = 0; v < nv; ++v) {
-        const int col = v0 + v;
-        const C x0 = Prec<C>::from(xb[(size_t)(3 * col) * B]), x1 = Prec<C>::from(xb[(size_t)(3 * col + 1) * B]),
-                x2 = Prec<C>::from(xb[(size_t)(3 * col + 2) * B]);
+        const C x0 = n0, x1 = n1, x2 = n2;
+        if (v + 1 < nv) {
+            const int col = v0 + v + 1;
+            n0 = xb[(size_t)(3 * col) * B];
+            n1 = xb[(size_t)(3 * col + 1) * B];
+            n2 = xb[(size_t)(3 * col + 2) * B];
+        }
 #pragma unroll
         for (int u = 0; u < NT; ++u) {
-            const C *m = sm + (u * VC + v) * 9;
-            cfma(acc[u][0], m[0], x0);
-            cfma(acc[u][0], m[3], x1);
-            cfma(acc[u][0], m[6], x2);
-            cfma(acc[u][1], m[1], x0);
-            cfma(acc[u][1], m[4], x1);
-            cfma(acc[u][1], m[7], x2);
-            cfma(acc[u][2], m[2], x0);
-            cfma(acc[u][2], m[5], x1);
-            cfma(acc[u][2], m[8], x2);
+            const cf *m = sm + (u * VC + v) * 9;  // column-major 3x3
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                cfma(acc[u][i], Prec<C>::from(m[i]), x0);
+                cfma(acc[u][i], Prec<C>::from(m[i + 3]), x1);
+                cfma(acc[u][i], Prec<C>::from(m[i + 6]), x2);
+            }
         }
     }
 }

@@ -35,19 +35,22 @@ static C to(cd a) {
 
 // one pass of NT row tets, lane by lane, barriers as phase boundaries
 template <typename C, int NT, int GT, int VC>
-static void emulate_pass(const GatherArgs<C> &g, const std::vector<Item> &it, int tp, std::vector<C> &sm) {
+static void emulate_pass(const GatherArgs<C> &g, const std::vector<Item> &it, int tp, std::vector<cd> &smbuf) {
+    // the kernel's staging buffer, viewed as the S* records (C) or the blocks (complex64)
+    C *sm = (C *)smbuf.data();
+    cf *smb = (cf *)smbuf.data();
     std::vector<std::array<std::array<C, 3>, NT>> acc(64);
     for (int l = 0; l < 64; ++l) zero_acc<C, NT>((C(*)[3])acc[l].data());
-    for (int l = 0; l < 64; ++l) stage_s4(g, tp, NT, l, sm.data());
-    for (int l = 0; l < 64; ++l) grid_part<C, NT>(g, it[l], sm.data(), (C(*)[3])acc[l].data());
+    for (int l = 0; l < 64; ++l) stage_s4(g, tp, NT, l, sm);
+    for (int l = 0; l < 64; ++l) grid_part<C, NT>(g, it[l], sm, (C(*)[3])acc[l].data());
     std::vector<int64_t> rsu(64);
     for (int l = 0; l < 64; ++l) rsu[l] = lane_row_start<C, GT, VC>(g, tp, NT, l);
     int q = 0;
     for_near_cubes(g, it[0], [&](int c0, int c1) {
         for (int v0 = c0; v0 < c1; v0 += VC) {
             const int nv = std::min(VC, c1 - v0);
-            for (int l = 0; l < 64; ++l) stage_blocks<C, GT, VC>(g, NT, nv, q, l, rsu[l], sm.data());
-            for (int l = 0; l < 64; ++l) chunk_part<C, NT, VC>(g, it[l], v0, nv, sm.data(), (C(*)[3])acc[l].data());
+            for (int l = 0; l < 64; ++l) stage_blocks<C, GT, VC>(g, NT, nv, q, l, rsu[l], smb);
+            for (int l = 0; l < 64; ++l) chunk_part<C, NT, VC>(g, it[l], v0, nv, smb, (C(*)[3])acc[l].data());
             q += nv;
         }
     });
@@ -57,7 +60,7 @@ static void emulate_pass(const GatherArgs<C> &g, const std::vector<Item> &it, in
 template <typename C, int GT, int VC>
 static void emulate(const GatherArgs<C> &g) {
     static_assert(GT == 8, "pass switch covers 1..8");
-    std::vector<C> sm(Shape<GT, VC>::LDS_N);
+    std::vector<cd> sm((Shape<C, GT, VC>::BYTES + 15) / 16);  // exact size, 16-byte aligned
     const long long nw = (long long)g.n_ne * ((g.B + 63) >> 6);
     for (long long b = 0; b < nw; ++b) {
         const long long w = xcd_remap(b, nw);  // the kernel's order (one item per workgroup)

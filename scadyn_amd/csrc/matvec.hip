@@ -465,7 +465,9 @@ __global__ __launch_bounds__(256) void k_gather_cube_w64(
 // each near-zone block once per wave instead of once per lane.
 template <typename C, int NT, int GT, int VC>
 __device__ __forceinline__ void gather_pass(const gcore::GatherArgs<C> &g, const gcore::Item &it, int tp, int lane,
-                                            C *sm) {
+                                            unsigned char *smraw) {
+    C *sm = (C *)smraw;         // S* records of the pass's tets
+    cf *smb = (cf *)smraw;      // then the near-zone blocks of each chunk
     using namespace gcore;
     C acc[NT][3];
     zero_acc<C, NT>(acc);
@@ -479,9 +481,9 @@ __device__ __forceinline__ void gather_pass(const gcore::GatherArgs<C> &g, const
         for (int v0 = c0; v0 < c1; v0 += VC) {
             const int nv = min(VC, c1 - v0);
             __syncthreads();
-            stage_blocks<C, GT, VC>(g, NT, nv, q, lane, rsu, sm);
+            stage_blocks<C, GT, VC>(g, NT, nv, q, lane, rsu, smb);
             __syncthreads();
-            chunk_part<C, NT, VC>(g, it, v0, nv, sm, acc);
+            chunk_part<C, NT, VC>(g, it, v0, nv, smb, acc);
             q += nv;
         }
     });
@@ -492,7 +494,7 @@ template <typename C, int GT, int VC>
 __global__ __launch_bounds__(64, 3) void k_gather_lds(gcore::GatherArgs<C> g) {
     using namespace gcore;
     static_assert(GT == 8, "the pass switch below covers 1..8 row tets");
-    __shared__ C sm[Shape<GT, VC>::LDS_N];
+    __shared__ __align__(16) unsigned char sm[Shape<C, GT, VC>::BYTES];
     const int lane = threadIdx.x;
     const long long nw = (long long)g.n_ne * ((g.B + 63) >> 6);
     for (long long b = blockIdx.x; b < nw; b += gridDim.x) {
