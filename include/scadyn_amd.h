@@ -85,7 +85,7 @@ typedef struct scadyn_opts {
                             with an in-pass Gram correction (same recurrence, about
                             half the HBM traffic of 0)                                  */
     int32_t fft_kernel;  /* line-FFT kernels: 0 automatic (register two-stage kernel where
-                            the axis length is R1*R2 with R1, R2 <= 16, else LDS
+                            the axis length is R1*R2 with R1, R2 <= 20, else LDS
                             Stockham), 1 LDS Stockham on every axis                     */
     int32_t mgs_block;   /* ortho 0: basis vectors whose MGS coefficients one pass over the
                             rows evaluates (in-pass Gram correction, same recurrence):

@@ -2,6 +2,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <condition_variable>
 #include <deque>
@@ -956,8 +958,28 @@ struct ColumnSolver {
             HIP_CHECK(hipMemcpyAsync(w->ret, h_ret.p, sizeof(int) * B, hipMemcpyHostToDevice, op->s));
             launch_store_x(d, w->ret, w->X, dst, ld, op->s);
             launch_retire(d, w->ret, w->st, op->s);
+            for (int b = 0; b < B; ++b) ndone += h_ret.p[b] >= 0;
         }
+        progress();
     }
+
+    // SCADYN_PROGRESS=<seconds>: a line on stderr at most that often while a
+    // long solve runs (systems finished, solver steps)
+    void progress() {
+        static const double every = [] {
+            const char *e = std::getenv("SCADYN_PROGRESS");
+            return e ? std::atof(e) : 0.0;
+        }();
+        if (every <= 0.0) return;
+        const auto now = std::chrono::steady_clock::now();
+        if (std::chrono::duration<double>(now - last_note).count() < every) return;
+        last_note = now;
+        std::fprintf(stderr, "[scadyn] %d of %zu systems done after %lld steps (batch %d)\n", ndone, cols.size(),
+                     steps, B);
+        std::fflush(stderr);
+    }
+    int ndone = 0;
+    std::chrono::steady_clock::time_point last_note = std::chrono::steady_clock::now();
 };
 
 // Solver settings of the operator's precision mode.  FP64: the reference's

@@ -293,10 +293,11 @@ void make_line_pass(LinePass &p, int L, int n_in, int n_out, long long inner, lo
     p.outer = outer;
     p.reg = 0;
     p.threads = 256;
-    if (allow_reg) {
+    // the register kernel indexes inner columns in 32 bits (convolution pass)
+    if (allow_reg && inner < (1LL << 31)) {
         // most balanced split L = R1 R2 (R1 >= R2) with both radices in the register set
         int best = 0;
-        for (int r1 = 16; r1 >= 2; --r1)
+        for (int r1 = 20; r1 >= 2; --r1)
             if (L % r1 == 0 && reg_radix_ok(r1) && reg_radix_ok(L / r1) && r1 >= L / r1) best = r1;
         if (best) {
             const int r1 = best, r2 = L / best, rmax = std::max(r1, r2);

@@ -49,17 +49,21 @@ void launch_line_reg(const LinePass &p, const C *in, C *out, const C *tw, int si
     if (blocks > 0x7fffffffLL) fail(SCADYN_ERR_UNSUPPORTED, "FFT grid too large");
     const size_t lds = line_pass_lds(p, sizeof(C));
     const dim3 grid((unsigned)blocks), block(p.threads);
-    const bool r12 = reg_rmax(p) == 12;
+    const int rm = reg_rmax(p);
     if (mode == 0) {
-        if (r12)
+        if (rm == 12)
             hipLaunchKernelGGL((k_line_reg<0, C, 12>), grid, block, lds, s, in, out, p, tw, sign, fg, ci);
-        else
+        else if (rm == 16)
             hipLaunchKernelGGL((k_line_reg<0, C, 16>), grid, block, lds, s, in, out, p, tw, sign, fg, ci);
-    } else {
-        if (r12)
-            hipLaunchKernelGGL((k_line_reg<1, C, 12>), grid, block, lds, s, in, out, p, tw, sign, fg, ci);
         else
+            hipLaunchKernelGGL((k_line_reg<0, C, 20>), grid, block, lds, s, in, out, p, tw, sign, fg, ci);
+    } else {
+        if (rm == 12)
+            hipLaunchKernelGGL((k_line_reg<1, C, 12>), grid, block, lds, s, in, out, p, tw, sign, fg, ci);
+        else if (rm == 16)
             hipLaunchKernelGGL((k_line_reg<1, C, 16>), grid, block, lds, s, in, out, p, tw, sign, fg, ci);
+        else
+            hipLaunchKernelGGL((k_line_reg<1, C, 20>), grid, block, lds, s, in, out, p, tw, sign, fg, ci);
     }
     HIP_CHECK(hipGetLastError());
 }

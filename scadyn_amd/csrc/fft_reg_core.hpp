@@ -1,4 +1,4 @@
-// Device (and host self-test) core of the two-stage register FFT for line lengths L = R1 * R2 with R1, R2 <= 16.
+// Device (and host self-test) core of the two-stage register FFT for line lengths L = R1 * R2 with R1, R2 <= 20.
 //
 // Each thread owns one radix-R sub-transform per stage and keeps it in VGPRs
 // (compile-time unrolled in-register DFTs with compile-time twiddles); the
@@ -230,10 +230,12 @@ COMPOSITE(12, 4, 3)
 COMPOSITE(14, 7, 2)
 COMPOSITE(15, 5, 3)
 COMPOSITE(16, 4, 4)
+COMPOSITE(18, 9, 2)
+COMPOSITE(20, 5, 4)
 #undef COMPOSITE
 
 // radices the register path instantiates (must match reg_radix_ok)
-#define REG_RADICES(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(12) X(14) X(15) X(16)
+#define REG_RADICES(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(12) X(14) X(15) X(16) X(18) X(20)
 
 struct Ctx {
     long long o;   // outer slab
@@ -241,7 +243,13 @@ struct Ctx {
     bool col_ok;
     int c;         // column within the tile
     int j;         // item index within the stage
+    int sh;        // log2 of the tile width
+    size_t in0;    // element (o, line point 0, i) of the input
+    size_t out0;   // element (o, line point 0, i) of the output
 };
+
+// Addressing is incremental: every stage walks its R elements with a running
+// pointer or index (one add each) instead of a product per element.
 
 // stage 1: radix R, x[j + r (L/R)] from HBM (zero beyond n_in), DFT, LDS
 template <int R, int SIGN, typename C>
@@ -249,15 +257,20 @@ HD void stage_first(const C *__restrict__ in, C *lds, const LinePass &p, const C
     const int LR = p.L / R;
     if (x.j >= LR) return;
     C v[R];
+    const size_t step = (size_t)LR * p.inner;
+    const C *src = in + x.in0 + (size_t)x.j * p.inner;
+    int n = x.j;
     sfor<0, R>([&](auto rr) {
         constexpr int r = CV(rr);
-        const int n = x.j + r * LR;
-        v[r] = (x.col_ok && n < p.n_in) ? in[((size_t)x.o * p.n_in + n) * p.inner + x.i] : czero<C>();
+        v[r] = (x.col_ok && n < p.n_in) ? *src : czero<C>();
+        src += step;
+        n += LR;
     });
     RDFT<R, SIGN, C>::run(v);
+    const int lb = ((x.j * R) << x.sh) + x.c;
     sfor<0, R>([&](auto rr) {
         constexpr int r = CV(rr);
-        lds[(x.j * R + r) * p.ti + x.c] = v[r];
+        lds[lb + (r << x.sh)] = v[r];
     });
 }
 
@@ -265,13 +278,18 @@ HD void stage_first(const C *__restrict__ in, C *lds, const LinePass &p, const C
 template <int R, int SIGN, typename C>
 HD void stage_second(C *v, const C *lds, const C *tw, const LinePass &p, const Ctx &x) {
     const int LR = p.L / R;
+    const int ls = LR << x.sh;
+    int li = (x.j << x.sh) + x.c;
     sfor<0, R>([&](auto rr) {
         constexpr int r = CV(rr);
-        v[r] = lds[(x.j + r * LR) * p.ti + x.c];
+        v[r] = lds[li];
+        li += ls;
     });
+    int ti = 0;  // j r
     sfor<1, R>([&](auto rr) {
         constexpr int r = CV(rr);
-        C w = tw[x.j * r];
+        ti += x.j;
+        C w = tw[ti];
         if constexpr (SIGN > 0) w.y = -w.y;
         v[r] = cmul(v[r], w);
     });
@@ -287,15 +305,21 @@ HD void stage_last(C *__restrict__ out, const C *lds, const C *tw, const LinePas
     C v[R];
     stage_second<R, SIGN, C>(v, lds, tw, p, x);
     if (!x.col_ok) return;
+    const size_t step = (size_t)LR * p.inner;
+    C *dst = out + x.out0 + (size_t)x.j * p.inner;
+    int k = x.j;
     sfor<0, R>([&](auto rr) {
         constexpr int r = CV(rr);
-        const int k = x.j + r * LR;
-        if (k < p.n_out) out[((size_t)x.o * p.n_out + k) * p.inner + x.i] = v[r];
+        if (k < p.n_out) *dst = v[r];
+        dst += step;
+        k += LR;
     });
 }
 
 // convolution middle, first half: forward stage 2 (radix R), x Fg, inverse
-// stage 1 (radix R) on the same registers
+// stage 1 (radix R) on the same registers.  The inner index is below 2^31
+// (make_line_pass keeps larger passes off the register kernel), so the grid
+// point is found with 32-bit divisions.
 template <int R, typename C>
 HD void conv_compute(C *v, const C *lds, const C *tw, const C *__restrict__ fg, const ConvInfo &ci, const LinePass &p,
                      const Ctx &x) {
@@ -303,16 +327,17 @@ HD void conv_compute(C *v, const C *lds, const C *tw, const C *__restrict__ fg, 
     if (x.j >= LR) return;
     stage_second<R, -1, C>(v, lds, tw, p, x);
     if (x.col_ok) {
-        const long long xy = x.i / ci.nb;
-        const int gx = (int)(xy % ci.Lx), gy = (int)(xy / ci.Lx);
+        const unsigned xy = (unsigned)x.i / (unsigned)ci.nb;
+        const int gy = (int)(xy / (unsigned)ci.Lx), gx = (int)(xy - (unsigned)gy * (unsigned)ci.Lx);
         const int kx = imin(gx, ci.Lx - gx), ky = imin(gy, ci.Ly - gy);
         const C *f = fg + (size_t)ky * ci.Hx + kx;
-        const size_t zs = (size_t)ci.Hy * ci.Hx;
+        const int zs = ci.Hy * ci.Hx;
+        int k = x.j;
         sfor<0, R>([&](auto rr) {
             constexpr int r = CV(rr);
-            const int k = x.j + r * LR;
             const int kz = imin(k, ci.Lz - k);
-            v[r] = cmul(v[r], f[kz * zs]);
+            v[r] = cmul(v[r], f[(size_t)kz * zs]);
+            k += LR;
         });
     }
     RDFT<R, +1, C>::run(v);
@@ -322,21 +347,31 @@ HD void conv_compute(C *v, const C *lds, const C *tw, const C *__restrict__ fg, 
 template <int R, typename C>
 HD void conv_store(const C *v, C *lds, const LinePass &p, const Ctx &x) {
     if (x.j >= p.L / R) return;
+    const int lb = ((x.j * R) << x.sh) + x.c;
     sfor<0, R>([&](auto rr) {
         constexpr int r = CV(rr);
-        lds[(x.j * R + r) * p.ti + x.c] = v[r];
+        lds[lb + (r << x.sh)] = v[r];
     });
 }
 
+HD int log2_pow2(int t) {
+    int s = 0;
+    while ((1 << s) < t) ++s;
+    return s;
+}
+
 HD Ctx make_ctx(const LinePass &p, long long block, int thread) {
-    const long long tiles = (p.inner + p.ti - 1) / p.ti;
     Ctx x;
+    x.sh = log2_pow2(p.ti);  // the tile width is a power of two
+    const long long tiles = (p.inner + p.ti - 1) >> x.sh;
     x.o = block / tiles;
-    const long long i0 = (block - x.o * tiles) * p.ti;
+    const long long i0 = (block - x.o * tiles) << x.sh;
     x.c = thread & (p.ti - 1);
-    x.j = thread / p.ti;
+    x.j = thread >> x.sh;
     x.i = i0 + x.c;
     x.col_ok = x.i < p.inner;
+    x.in0 = (size_t)x.o * p.n_in * (size_t)p.inner + (size_t)x.i;
+    x.out0 = (size_t)x.o * p.n_out * (size_t)p.inner + (size_t)x.i;
     return x;
 }
 
@@ -344,7 +379,7 @@ HD Ctx make_ctx(const LinePass &p, long long block, int thread) {
 // of sign s): phase 0 = stage 1, phase 1 = stage 2.  MODE 1 (forward, x Fg,
 // inverse): phase 0 = forward stage 1, phase 1 = conv_compute (registers v),
 // phase 2 = conv_store, phase 3 = inverse stage 2.
-template <int MODE, typename C, int RMAX = 16>
+template <int MODE, typename C, int RMAX = 20>
 HD void line_reg_phase(int phase, const C *__restrict__ in, C *__restrict__ out, C *lds, const C *tw,
                        const C *__restrict__ fg, const ConvInfo &ci, const LinePass &p, const Ctx &x, int sign,
                        C *v) {
@@ -399,8 +434,11 @@ HD void line_reg_phase(int phase, const C *__restrict__ in, C *__restrict__ out,
     }
 }
 
-// smallest compiled radix bound covering a pass
-HD int reg_rmax(const LinePass &p) { return (p.rad[0] <= 12 && p.rad[1] <= 12) ? 12 : 16; }
+// smallest compiled radix bound covering a pass (instances 12, 16 and 20)
+HD int reg_rmax(const LinePass &p) {
+    const int r = p.rad[0] > p.rad[1] ? p.rad[0] : p.rad[1];
+    return r <= 12 ? 12 : (r <= 16 ? 16 : 20);
+}
 
 }  // namespace regfft
 }  // namespace scadyn

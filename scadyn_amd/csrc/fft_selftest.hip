@@ -40,17 +40,20 @@ template <int MODE, typename C>
 static void emulate(const LinePass &p, const C *in, C *out, const C *twg, int sign, const C *fg, const ConvInfo &ci) {
     const long long tiles = (p.inner + p.ti - 1) / p.ti, blocks = tiles * p.outer;
     std::vector<C> lds((size_t)p.L * p.ti), tw(twg, twg + p.L);
-    std::vector<std::array<C, 16>> regs(p.threads);
+    std::vector<std::array<C, 20>> regs(p.threads);
     const int nph = MODE == 0 ? 2 : 4;
-    const bool r12 = reg_rmax(p) == 12;  // the instance the launcher picks
+    const int rm = reg_rmax(p);  // the instance the launcher picks
     for (long long b = 0; b < blocks; ++b)
         for (int ph = 0; ph < nph; ++ph)
             for (int t = 0; t < p.threads; ++t) {
-                if (r12)
+                if (rm == 12)
                     line_reg_phase<MODE, C, 12>(ph, in, out, lds.data(), tw.data(), fg, ci, p, make_ctx(p, b, t),
                                                 sign, regs[t].data());
-                else
+                else if (rm == 16)
                     line_reg_phase<MODE, C, 16>(ph, in, out, lds.data(), tw.data(), fg, ci, p, make_ctx(p, b, t),
+                                                sign, regs[t].data());
+                else
+                    line_reg_phase<MODE, C, 20>(ph, in, out, lds.data(), tw.data(), fg, ci, p, make_ctx(p, b, t),
                                                 sign, regs[t].data());
             }
 }
@@ -157,7 +160,7 @@ static void conv_case(int Lz, std::mt19937_64 &rng) {
 
 int main() {
     std::mt19937_64 rng(12345);
-    const int rad[] = {2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 14, 15, 16};
+    const int rad[] = {2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 14, 15, 16, 18, 20};
     std::vector<int> lengths;
     for (int a : rad)
         for (int b : rad) {

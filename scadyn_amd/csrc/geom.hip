@@ -7,6 +7,7 @@
 #include <cstring>
 
 #include "internal.hpp"
+#include "fft.hpp"
 
 namespace scadyn {
 
@@ -288,13 +289,30 @@ std::vector<cd> pinv_jacobi(const std::vector<cd> &A0, int m, int n, double thr)
     return P;
 }
 
+static bool is_smooth(int L) {
+    int r = L;
+    for (int p : {2, 3, 5, 7})
+        while (r % p == 0) r /= p;
+    return r == 1;
+}
+
+// L = R1 R2 with both radices in the register FFT's set
+static bool reg_splittable(int L) {
+    for (int r1 = 2; r1 <= 20; ++r1)
+        if (L % r1 == 0 && reg_radix_ok(r1) && reg_radix_ok(L / r1)) return true;
+    return false;
+}
+
+// FFT length for a zero-padded axis of n >= 2N-1 points: the smallest
+// 2,3,5,7-smooth length, or a longer one (at most n/8 more points) that the
+// two-stage register FFT covers when the smallest does not
 int smooth_length(int n) {
-    for (int L = std::max(n, 1);; ++L) {
-        int r = L;
-        for (int p : {2, 3, 5, 7})
-            while (r % p == 0) r /= p;
-        if (r == 1) return L;
-    }
+    int s0 = std::max(n, 1);
+    while (!is_smooth(s0)) ++s0;
+    if (reg_splittable(s0)) return s0;
+    for (int L = s0 + 1; L <= n + n / 8; ++L)
+        if (is_smooth(L) && reg_splittable(L)) return L;
+    return s0;
 }
 
 std::vector<int> factor_radices(int L) {

@@ -62,3 +62,27 @@ def test_tuned_tiles():
         assert rel(ya[r], yb[r]) < 2e-6
     a.close()
     b.close()
+
+
+def test_register_radix_18_20():
+    """Axis lengths above 256 (the 824k-tet sphere: 132 nodes per axis, L = 270
+    = 15 x 18) take the radix-18/20 register kernel; it agrees with the LDS
+    Stockham kernel through FFT(G) and the matvec in both precisions."""
+    from scadyn_amd import native
+    c, e, param, k = sphere_case(n_cells=64)
+    a = native.Operator(c, e, param, k, precision=native.PREC_MIXED)
+    inf = a.info()
+    say(f"{e.shape[0]} tets, fft {inf['Lx']}x{inf['Ly']}x{inf['Lz']}")
+    assert max(inf['Lx'], inf['Ly'], inf['Lz']) > 256
+    b = native.Operator(c, e, param, k, precision=native.PREC_MIXED, fft_kernel=1)
+    assert rel(a.export('fg'), b.export('fg')) < 1e-13
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal((3, 3 * e.shape[0])) + 1j * rng.standard_normal((3, 3 * e.shape[0]))
+    ya, yb = a.matvec(x), b.matvec(x)
+    for r in range(3):
+        assert rel(ya[r], yb[r]) < 1e-13
+    ya, yb = a.matvec(x, precision=native.PREC_MIXED), b.matvec(x, precision=native.PREC_MIXED)
+    for r in range(3):
+        assert rel(ya[r], yb[r]) < 2e-6
+    a.close()
+    b.close()
