@@ -151,6 +151,7 @@ struct Workspace {
     // complex64 (Xs [n][B], read with zero offsets zoff)
     cf *Xs = nullptr;
     long long *zoff = nullptr;
+    int *ract = nullptr;  // mixed precision: restarting slots of a full-width FP64 matvec
     SlotState *st = nullptr;
     cd *H = nullptr, *cs = nullptr, *sn = nullptr, *g = nullptr, *y = nullptr;
     double *nrm2 = nullptr, *nav2 = nullptr, *pnrm = nullptr, *pnav = nullptr;
@@ -464,11 +465,12 @@ static Workspace &workspace(scadyn_operator *op, int B) {
     } else {
         w.V = w.mem.alloc<cd>((size_t)(m + 2) * nB);
         w.X = w.V + (size_t)(m + 1) * nB;
-        if (op->opts.precision == SCADYN_PREC_MIXED) {
-            w.Xs = w.mem.alloc<cf>(nB);
-            w.zoff = w.mem.alloc<long long>(B);
-            HIP_CHECK(hipMemsetAsync(w.zoff, 0, sizeof(long long) * B, op->s));
-        }
+        if (op->opts.precision == SCADYN_PREC_MIXED) w.Xs = w.mem.alloc<cf>(nB);
+    }
+    if (op->opts.precision == SCADYN_PREC_MIXED) {
+        w.zoff = w.mem.alloc<long long>(B);
+        HIP_CHECK(hipMemsetAsync(w.zoff, 0, sizeof(long long) * B, op->s));
+        w.ract = w.mem.alloc<int>(B);
     }
     w.Rhs = w.mem.alloc<cd>(nB);
     w.W = w.mem.alloc<cd>(nB);
@@ -791,7 +793,7 @@ struct ColumnSolver {
     KrylovDims d;
     int B = 0;
     Pinned<SlotState> hs;
-    Pinned<int> h_load, h_ret, h_rs;
+    Pinned<int> h_load, h_ret, h_rs, h_ract;
     std::vector<int> slot_col;
     int next = 0, nfail = 0, singular = 0, max_cycles = 0;
     long long steps = 0, max_steps = 0;
@@ -826,6 +828,7 @@ struct ColumnSolver {
         if (spec.verify64) {
             restart_space(op, *w);
             h_rs.alloc(((B + w->rs_cap - 1) / w->rs_cap) * w->rs_cap);
+            h_ract.alloc(B);
         }
         std::memset(hs.p, 0, sizeof(SlotState) * B);
         slot_col.assign(B, -1);
@@ -895,6 +898,16 @@ struct ColumnSolver {
                 } else {
                     run_matvec<cd>(op, w->mv, w->V, w->mv.active, w->W);
                 }
+            }
+            // many restarting slots (a synchronised restart of the batch): one
+            // full-width FP64 matvec of x over the batch grids, masked to them,
+            // costs less than nrst / rs_cap narrow ones (about half per slot)
+            if (nrst > 0 && 2 * nrst >= B && w->ract) {
+                for (int b = 0; b < B; ++b) h_ract.p[b] = 0;
+                for (int j = 0; j < nrst; ++j) h_ract.p[h_rs.p[j]] = 1;
+                HIP_CHECK(hipMemcpyAsync(w->ract, h_ract.p, sizeof(int) * B, hipMemcpyHostToDevice, s));
+                run_matvec<cd>(op, w->mv, w->X, w->ract, w->W, w->zoff);
+                nrst = 0;
             }
             for (int r0 = 0; r0 < nrst; r0 += w->rs_cap) {
                 const int nr = std::min(w->rs_cap, nrst - r0);
