@@ -146,7 +146,7 @@ struct Workspace {
     // there), or for the mixed-precision solve (ortho 0, mgs_block > 1) a
     // complex64 basis Vf [(m+1)][n][B] and x in its own complex128 X
     cd *V = nullptr, *Rhs = nullptr, *W = nullptr, *X = nullptr;
-    cf *Vf = nullptr;
+    cf *Vf = nullptr, *Wf = nullptr;  // complex64 basis and orthogonalised w (mixed precision)
     // mixed precision with a complex128 basis: the batch's inputs converted to
     // complex64 (Xs [n][B], read with zero offsets zoff)
     cf *Xs = nullptr;
@@ -421,7 +421,11 @@ static void create_operator(scadyn_operator *op, const scadyn_mesh &mesh, const 
 static size_t slot_bytes(const scadyn_operator *op, int m) {
     const size_t n = 3 * (size_t)op->g.N;
     const size_t grid = (size_t)op->L[0] * op->L[1] * op->g.Nz * 4;
-    return sizeof(cd) * ((size_t)(m + 4) * n + 2 * grid);
+    // complex64 basis Vf and w copy with complex128 x, W, Rhs; or the complex128
+    // basis with x, W, Rhs and one vector of slack
+    const size_t vec = f32_basis(op) ? (size_t)(m + 1) * n * sizeof(cf) + n * (3 * sizeof(cd) + sizeof(cf))
+                                     : (size_t)(m + 4) * n * sizeof(cd);
+    return vec + 2 * grid * sizeof(cd);  // two grids (complex128 size: both precisions)
 }
 
 static void make_matvec_space(scadyn_operator *op, int B, MatvecSpace &ms) {
@@ -461,6 +465,7 @@ static Workspace &workspace(scadyn_operator *op, int B) {
     tune_fft_tiles(op, w.mv);
     if (f32_basis(op)) {
         w.Vf = w.mem.alloc<cf>((size_t)(m + 1) * nB);
+        w.Wf = w.mem.alloc<cf>(nB);
         w.X = w.mem.alloc<cd>(nB);
     } else {
         w.V = w.mem.alloc<cd>((size_t)(m + 2) * nB);
@@ -742,6 +747,9 @@ static int auto_batch(scadyn_operator *op, int ncols) {
         if ((Bg % 64 == 0 || Bg % 64 >= 48) && (size_t)Bg <= cap) B = Bg;
     }
     B = (int)std::min<size_t>((size_t)B, cap);
+    // a memory-capped batch keeps the slot-chunk kernels: round down to a width
+    // whose last 64-slot chunk holds at least 48 slots
+    if (B > 64 && B % 64 != 0 && B % 64 < 48) B -= B % 64;
     return std::max(1, B);
 }
 
@@ -925,8 +933,8 @@ struct ColumnSolver {
                 launch_bmgs(d, w->W, w->V, w->Rhs, w->st, w->H, w->pdot, w->pnrm, w->pnav, w->cc, w->nrm2, w->nav2,
                             std::max(K - 1, 0), s);
             } else if (spec.mgs_block > 1 && w->Vf) {
-                launch_mgsb(d, spec.mgs_block, K, w->W, (const cf *)w->Vf, w->Rhs, w->st, w->H, w->pdot, w->pnrm,
-                            w->pnav, w->nrm2, w->nav2, s);
+                launch_mgsb(d, spec.mgs_block, K, w->W, w->Wf, (const cf *)w->Vf, w->Rhs, w->st, w->H, w->pdot,
+                            w->pnrm, w->pnav, w->nrm2, w->nav2, s);
             } else if (spec.mgs_block > 1) {
                 launch_mgsb(d, spec.mgs_block, K, w->W, (const cd *)w->V, w->Rhs, w->st, w->H, w->pdot, w->pnrm,
                             w->pnav, w->nrm2, w->nav2, s);
@@ -939,7 +947,7 @@ struct ColumnSolver {
             launch_slot_update(d, w->st, w->H, w->cs, w->sn, w->g, w->y, w->nrm2, w->nav2, spec.tol, spec.restart,
                                spec.verify64 ? 1 : 0, spec.tol_true, s);
             if (w->Vf)
-                launch_vec_update(d, w->st, w->W, w->Vf, w->X, w->y, s);
+                launch_vec_update(d, w->st, (const cf *)w->Wf, w->Vf, w->X, w->y, s);
             else
                 launch_vec_update(d, w->st, w->W, w->V, w->X, w->y, s);
         }

@@ -147,8 +147,8 @@ __global__ __launch_bounds__(RED_T) void k_mgs_reduce(KrylovDims d, int k, const
 // ---- MGS evaluated BS basis vectors per pass (opts.ortho = 0, mgs_block > 1) ----
 // The per-thread row sweep and the per-slot coefficient step are in
 // mgsb_core.hpp (shared with the host self-test).
-template <int BS, typename VT>
-__global__ __launch_bounds__(256) void k_mgsb_pass(KrylovDims d, int p, cd *__restrict__ W,
+template <int BS, typename VT, typename WI, typename WO>
+__global__ __launch_bounds__(256) void k_mgsb_pass(KrylovDims d, int p, const WI *Win, WO *Wout,
                                                    const VT *__restrict__ V, const cd *__restrict__ Rhs,
                                                    const SlotState *__restrict__ st, const cd *__restrict__ H,
                                                    cd *__restrict__ pdot, double *__restrict__ pnrm,
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(256) void k_mgsb_pass(KrylovDims d, int p, cd *__re
     for (int q = 0; q < NQ; ++q) acc[q] = cmk(0.0, 0.0);
     double nrm = 0.0, nav = 0.0;
     if (m.ok)
-        mgsb::sweep_rows<BS, VT>(d, p, W, V, Rhs, st, H, m.b, (long long)blockIdx.x * m.RW + m.rl,
+        mgsb::sweep_rows<BS, VT, WI, WO>(d, p, Win, Wout, V, Rhs, st, H, m.b, (long long)blockIdx.x * m.RW + m.rl,
                              (long long)gridDim.x * m.RW, acc, nrm, nav);
     snrm[threadIdx.x] = nrm;
     snav[threadIdx.x] = nav;
@@ -210,14 +210,25 @@ __global__ __launch_bounds__(RED_T) void k_mgsb_reduce(KrylovDims d, int p, cons
     if (threadIdx.x == 0) mgsb::slot_coef<BS>(d, p, st, red, x1, x2, H, nrm2, nav2, b);
 }
 
+// passes 0 .. P; w in Wd (complex128) throughout, or (Wf given) read from Wd
+// in passes 0 and 1 and kept in the complex64 Wf from the first write on
 template <int BS, typename VT>
-static void launch_mgsb_t(const KrylovDims &d, int K, cd *W, const VT *V, const cd *Rhs, const SlotState *st,
-                          cd *H, cd *pdot, double *pnrm, double *pnav, double *nrm2, double *nav2, hipStream_t s) {
+static void launch_mgsb_t(const KrylovDims &d, int K, cd *Wd, cf *Wf, const VT *V, const cd *Rhs,
+                          const SlotState *st, cd *H, cd *pdot, double *pnrm, double *pnav, double *nrm2,
+                          double *nav2, hipStream_t s) {
     const int P = (K + BS - 1) / BS;  // blocks of the vectors 0 .. K-1; pass P takes the last norm
     const int SB = d.B < 256 ? d.B : 256;
     const dim3 grid((unsigned)d.nwg, (unsigned)((d.B + SB - 1) / SB));
     for (int p = 0; p <= P; ++p) {
-        hipLaunchKernelGGL((k_mgsb_pass<BS, VT>), grid, dim3(256), 0, s, d, p, W, V, Rhs, st, H, pdot, pnrm, pnav);
+        if (!Wf)
+            hipLaunchKernelGGL((k_mgsb_pass<BS, VT, cd, cd>), grid, dim3(256), 0, s, d, p, (const cd *)Wd, Wd, V, Rhs,
+                               st, H, pdot, pnrm, pnav);
+        else if (p <= 1)
+            hipLaunchKernelGGL((k_mgsb_pass<BS, VT, cd, cf>), grid, dim3(256), 0, s, d, p, (const cd *)Wd, Wf, V, Rhs,
+                               st, H, pdot, pnrm, pnav);
+        else
+            hipLaunchKernelGGL((k_mgsb_pass<BS, VT, cf, cf>), grid, dim3(256), 0, s, d, p, (const cf *)Wf, Wf, V, Rhs,
+                               st, H, pdot, pnrm, pnav);
         hipLaunchKernelGGL((k_mgsb_reduce<BS>), dim3((unsigned)d.B), dim3(RED_T), 0, s, d, p, st, H, pdot, pnrm,
                            pnav, nrm2, nav2);
     }
@@ -225,23 +236,23 @@ static void launch_mgsb_t(const KrylovDims &d, int K, cd *W, const VT *V, const 
 }
 
 template <typename VT>
-static void launch_mgsb_v(const KrylovDims &d, int bs, int K, cd *W, const VT *V, const cd *Rhs,
+static void launch_mgsb_v(const KrylovDims &d, int bs, int K, cd *Wd, cf *Wf, const VT *V, const cd *Rhs,
                           const SlotState *st, cd *H, cd *pdot, double *pnrm, double *pnav, double *nrm2,
                           double *nav2, hipStream_t s) {
     switch (bs) {
-        case 2: launch_mgsb_t<2, VT>(d, K, W, V, Rhs, st, H, pdot, pnrm, pnav, nrm2, nav2, s); break;
-        case 4: launch_mgsb_t<4, VT>(d, K, W, V, Rhs, st, H, pdot, pnrm, pnav, nrm2, nav2, s); break;
-        case 8: launch_mgsb_t<8, VT>(d, K, W, V, Rhs, st, H, pdot, pnrm, pnav, nrm2, nav2, s); break;
+        case 2: launch_mgsb_t<2, VT>(d, K, Wd, Wf, V, Rhs, st, H, pdot, pnrm, pnav, nrm2, nav2, s); break;
+        case 4: launch_mgsb_t<4, VT>(d, K, Wd, Wf, V, Rhs, st, H, pdot, pnrm, pnav, nrm2, nav2, s); break;
+        case 8: launch_mgsb_t<8, VT>(d, K, Wd, Wf, V, Rhs, st, H, pdot, pnrm, pnav, nrm2, nav2, s); break;
         default: fail(SCADYN_ERR_ARG, "MGS block size must be 2, 4 or 8");
     }
 }
 void launch_mgsb(const KrylovDims &d, int bs, int K, cd *W, const cd *V, const cd *Rhs, const SlotState *st, cd *H,
                  cd *pdot, double *pnrm, double *pnav, double *nrm2, double *nav2, hipStream_t s) {
-    launch_mgsb_v<cd>(d, bs, K, W, V, Rhs, st, H, pdot, pnrm, pnav, nrm2, nav2, s);
+    launch_mgsb_v<cd>(d, bs, K, W, nullptr, V, Rhs, st, H, pdot, pnrm, pnav, nrm2, nav2, s);
 }
-void launch_mgsb(const KrylovDims &d, int bs, int K, cd *W, const cf *V, const cd *Rhs, const SlotState *st, cd *H,
-                 cd *pdot, double *pnrm, double *pnav, double *nrm2, double *nav2, hipStream_t s) {
-    launch_mgsb_v<cf>(d, bs, K, W, V, Rhs, st, H, pdot, pnrm, pnav, nrm2, nav2, s);
+void launch_mgsb(const KrylovDims &d, int bs, int K, cd *W, cf *Wf, const cf *V, const cd *Rhs, const SlotState *st,
+                 cd *H, cd *pdot, double *pnrm, double *pnav, double *nrm2, double *nav2, hipStream_t s) {
+    launch_mgsb_v<cf>(d, bs, K, W, Wf, V, Rhs, st, H, pdot, pnrm, pnav, nrm2, nav2, s);
 }
 int mgsb_partials(int bs) { return bs + bs * (bs - 1) / 2; }
 
@@ -563,9 +574,9 @@ __global__ void k_slot_update(KrylovDims d, SlotState *__restrict__ st, cd *__re
 }
 
 // new basis vector v = w / h (stored as VT), or at a cycle's end x += V y
-template <typename VT>
+template <typename VT, typename WT>
 __global__ __launch_bounds__(256) void k_vec_update(KrylovDims d, const SlotState *__restrict__ st,
-                                                    const cd *__restrict__ W, VT *__restrict__ V,
+                                                    const WT *__restrict__ W, VT *__restrict__ V,
                                                     cd *__restrict__ X, const cd *__restrict__ yall) {
     const size_t nB = (size_t)d.n * d.B;
     const long long tot = d.n * d.B;
@@ -575,7 +586,7 @@ __global__ __launch_bounds__(256) void k_vec_update(KrylovDims d, const SlotStat
         const SlotState &s = st[b];
         if (s.act == ACT_V0 || s.act == ACT_NEXT) {
             const double h = (s.act == ACT_V0) ? s.beta : s.hnorm;
-            cd w = W[e];
+            cd w = Prec<cd>::from(W[e]);
             if (h != 0.0) w = cmk(w.x / h, w.y / h);
             V[(size_t)s.tgt * nB + e] = Prec<VT>::from(w);
         } else if (s.act == ACT_XUPD) {
@@ -801,21 +812,21 @@ void launch_slot_update(const KrylovDims &d, SlotState *st, cd *H, cd *cs, cd *s
                        tol, restart, verify, tol_true);
     HIP_CHECK(hipGetLastError());
 }
-template <typename VT>
-static void launch_vec_update_t(const KrylovDims &d, const SlotState *st, const cd *W, VT *V, cd *X, const cd *y,
+template <typename VT, typename WT>
+static void launch_vec_update_t(const KrylovDims &d, const SlotState *st, const WT *W, VT *V, cd *X, const cd *y,
                                 hipStream_t s) {
     long long tot = d.n * d.B;
     unsigned g = (unsigned)std::min<long long>((tot + 255) / 256, 1 << 16);
-    hipLaunchKernelGGL((k_vec_update<VT>), dim3(g), dim3(256), 0, s, d, st, W, V, X, y);
+    hipLaunchKernelGGL((k_vec_update<VT, WT>), dim3(g), dim3(256), 0, s, d, st, W, V, X, y);
     HIP_CHECK(hipGetLastError());
 }
 void launch_vec_update(const KrylovDims &d, const SlotState *st, const cd *W, cd *V, cd *X, const cd *y,
                        hipStream_t s) {
-    launch_vec_update_t<cd>(d, st, W, V, X, y, s);
+    launch_vec_update_t<cd, cd>(d, st, W, V, X, y, s);
 }
-void launch_vec_update(const KrylovDims &d, const SlotState *st, const cd *W, cf *V, cd *X, const cd *y,
+void launch_vec_update(const KrylovDims &d, const SlotState *st, const cf *W, cf *V, cd *X, const cd *y,
                        hipStream_t s) {
-    launch_vec_update_t<cf>(d, st, W, V, X, y, s);
+    launch_vec_update_t<cf, cf>(d, st, W, V, X, y, s);
 }
 void launch_load_rhs(const KrylovDims &d, const int *col, const cd *src, long long ld, cd *Rhs, cd *X,
                      double *pnrm, int x0zero, hipStream_t s) {

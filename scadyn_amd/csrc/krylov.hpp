@@ -48,7 +48,7 @@ void launch_slot_update(const KrylovDims &d, SlotState *st, cd *H, cd *cs, cd *s
 // basis V stored as complex128 or (mixed precision) complex64; x in X (n x B)
 void launch_vec_update(const KrylovDims &d, const SlotState *st, const cd *W, cd *V, cd *X, const cd *y,
                        hipStream_t s);
-void launch_vec_update(const KrylovDims &d, const SlotState *st, const cd *W, cf *V, cd *X, const cd *y,
+void launch_vec_update(const KrylovDims &d, const SlotState *st, const cf *W, cf *V, cd *X, const cd *y,
                        hipStream_t s);
 // load column col[b] (>= 0) of src (n x ld, row-major) into Rhs and X of slot b
 void launch_load_rhs(const KrylovDims &d, const int *col, const cd *src, long long ld, cd *Rhs, cd *X,
@@ -76,8 +76,10 @@ int bmgs_partials();
 // of i+1 (0 when only restarting slots).  pdot holds nwg*mgsb_partials(bs)*B.
 void launch_mgsb(const KrylovDims &d, int bs, int K, cd *W, const cd *V, const cd *Rhs, const SlotState *st, cd *H,
                  cd *pdot, double *pnrm, double *pnav, double *nrm2, double *nav2, hipStream_t s);
-void launch_mgsb(const KrylovDims &d, int bs, int K, cd *W, const cf *V, const cd *Rhs, const SlotState *st, cd *H,
-                 cd *pdot, double *pnrm, double *pnav, double *nrm2, double *nav2, hipStream_t s);
+// mixed precision: complex64 basis, and w kept in the complex64 Wf once the
+// first pass that changes it has run (read from the complex128 W before)
+void launch_mgsb(const KrylovDims &d, int bs, int K, cd *W, cf *Wf, const cf *V, const cd *Rhs, const SlotState *st,
+                 cd *H, cd *pdot, double *pnrm, double *pnav, double *nrm2, double *nav2, hipStream_t s);
 int mgsb_partials(int bs);
 
 // matvec input offsets / activity derived on the device from the slot phases;

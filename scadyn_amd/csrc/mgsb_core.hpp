@@ -61,10 +61,14 @@ MGSB_HD PassRole pass_role(const SlotState &s, int p) {
 // one thread's rows (row0, row0 + rstep, ...) of slot b: partial dots/Gram in
 // acc[NQ], |w|^2 after the pass's update in nrm, before orthogonalisation in nav.
 // The basis is stored as VT (complex128, or complex64 for the mixed-precision
-// solve); all arithmetic is FP64.
-template <int BS, typename VT>
-MGSB_HD void sweep_rows(const KrylovDims &d, int p, cd *W, const VT *V, const cd *Rhs, const SlotState *st,
-                        const cd *H, int b, long long row0, long long rstep, cd *acc, double &nrm, double &nav) {
+// solve); all arithmetic is FP64.  w is read from Win (WI) and, where the pass
+// changes it, written to Wout (WO): the same complex128 array, or for the
+// mixed-precision solve the complex128 matvec output in, a complex64 copy out
+// from the first pass that writes w on (passes 0 and 1), then the copy alone.
+template <int BS, typename VT, typename WI, typename WO>
+MGSB_HD void sweep_rows(const KrylovDims &d, int p, const WI *Win, WO *Wout, const VT *V, const cd *Rhs,
+                        const SlotState *st, const cd *H, int b, long long row0, long long rstep, cd *acc,
+                        double &nrm, double &nav) {
     constexpr int NQ = Blk<BS>::NQ;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) acc[q] = cmk(0.0, 0.0);
@@ -79,10 +83,10 @@ MGSB_HD void sweep_rows(const KrylovDims &d, int p, cd *W, const VT *V, const cd
         h[u] = u < r.nu ? H[(size_t)b * (d.m + 1) * d.m + (size_t)(d.m + 1) * s.i + (r.ulo + u)] : cmk(0.0, 0.0);
     for (long long row = row0; row < d.n; row += rstep) {
         const size_t e = (size_t)row * d.B + b;
-        cd w = W[e];
+        cd w = Prec<cd>::from(Win[e]);
         if (r.rst) {
             w = csub(Rhs[e], w);
-            W[e] = w;
+            Wout[e] = Prec<WO>::from(w);
             nrm += cabs2(w);
             continue;
         }
@@ -91,7 +95,7 @@ MGSB_HD void sweep_rows(const KrylovDims &d, int p, cd *W, const VT *V, const cd
 #pragma unroll
             for (int u = 0; u < BS; ++u)
                 if (u < r.nu) w = csub(w, cmul(h[u], Prec<cd>::from(V[(size_t)(r.ulo + u) * nB + e])));
-            W[e] = w;
+            Wout[e] = Prec<WO>::from(w);
         }
         if (r.nd > 0) {
             cd v[BS];
