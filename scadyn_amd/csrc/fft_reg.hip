@@ -53,6 +53,8 @@ void launch_line_reg(const LinePass &p, const C *in, C *out, const C *tw, int si
     if (mode == 0) {
         if (rm == 12)
             hipLaunchKernelGGL((k_line_reg<0, C, 12>), grid, block, lds, s, in, out, p, tw, sign, fg, ci);
+        else if (rm == 14)
+            hipLaunchKernelGGL((k_line_reg<0, C, 14>), grid, block, lds, s, in, out, p, tw, sign, fg, ci);
         else if (rm == 16)
             hipLaunchKernelGGL((k_line_reg<0, C, 16>), grid, block, lds, s, in, out, p, tw, sign, fg, ci);
         else
@@ -60,6 +62,8 @@ void launch_line_reg(const LinePass &p, const C *in, C *out, const C *tw, int si
     } else {
         if (rm == 12)
             hipLaunchKernelGGL((k_line_reg<1, C, 12>), grid, block, lds, s, in, out, p, tw, sign, fg, ci);
+        else if (rm == 14)
+            hipLaunchKernelGGL((k_line_reg<1, C, 14>), grid, block, lds, s, in, out, p, tw, sign, fg, ci);
         else if (rm == 16)
             hipLaunchKernelGGL((k_line_reg<1, C, 16>), grid, block, lds, s, in, out, p, tw, sign, fg, ci);
         else

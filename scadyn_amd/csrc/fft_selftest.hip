@@ -49,6 +49,9 @@ static void emulate(const LinePass &p, const C *in, C *out, const C *twg, int si
                 if (rm == 12)
                     line_reg_phase<MODE, C, 12>(ph, in, out, lds.data(), tw.data(), fg, ci, p, make_ctx(p, b, t),
                                                 sign, regs[t].data());
+                else if (rm == 14)
+                    line_reg_phase<MODE, C, 14>(ph, in, out, lds.data(), tw.data(), fg, ci, p, make_ctx(p, b, t),
+                                                sign, regs[t].data());
                 else if (rm == 16)
                     line_reg_phase<MODE, C, 16>(ph, in, out, lds.data(), tw.data(), fg, ci, p, make_ctx(p, b, t),
                                                 sign, regs[t].data());
