@@ -152,6 +152,7 @@ struct Workspace {
     cf *Xs = nullptr;
     long long *zoff = nullptr;
     int *ract = nullptr;  // mixed precision: restarting slots of a full-width FP64 matvec
+    int *ierr = nullptr;  // set by the device when a matvec input offset was out of range
     SlotState *st = nullptr;
     cd *H = nullptr, *cs = nullptr, *sn = nullptr, *g = nullptr, *y = nullptr;
     double *nrm2 = nullptr, *nav2 = nullptr, *pnrm = nullptr, *pnav = nullptr;
@@ -479,6 +480,8 @@ static Workspace &workspace(scadyn_operator *op, int B) {
     }
     w.Rhs = w.mem.alloc<cd>(nB);
     w.W = w.mem.alloc<cd>(nB);
+    w.ierr = w.mem.alloc<int>(1);
+    HIP_CHECK(hipMemsetAsync(w.ierr, 0, sizeof(int), op->s));
     w.st = w.mem.alloc<SlotState>(B);
     HIP_CHECK(hipMemsetAsync(w.st, 0, sizeof(SlotState) * B, op->s));
     w.H = w.mem.alloc<cd>((size_t)B * (m + 1) * m);
@@ -801,7 +804,7 @@ struct ColumnSolver {
     KrylovDims d;
     int B = 0;
     Pinned<SlotState> hs;
-    Pinned<int> h_load, h_ret, h_rs, h_ract;
+    Pinned<int> h_load, h_ret, h_rs, h_ract, h_err;
     std::vector<int> slot_col;
     int next = 0, nfail = 0, singular = 0, max_cycles = 0;
     long long steps = 0, max_steps = 0;
@@ -831,6 +834,8 @@ struct ColumnSolver {
         d.m = w->m;
         d.nwg = w->nwg;
         hs.alloc(B);
+        h_err.alloc(1);
+        h_err.p[0] = 0;
         h_load.alloc(B);
         h_ret.alloc(B);
         if (spec.verify64) {
@@ -896,7 +901,8 @@ struct ColumnSolver {
                 if (hs.p[b].phase == PH_RESTART && spec.verify64) h_rs.p[nrst++] = b;
                 if (hs.p[b].phase == PH_ARNOLDI || (hs.p[b].phase == PH_RESTART && !spec.verify64)) ++narn;
             }
-            launch_set_inputs(d, w->st, w->mv.in_off, w->mv.active, spec.verify64 ? 1 : 0, s);
+            launch_set_inputs(d, w->st, w->mv.in_off, w->mv.active, spec.verify64 ? 1 : 0,
+                              w->Vf ? w->m + 1 : w->m + 2, w->ierr, s);
             if (narn > 0) {
                 if (spec.prec == SCADYN_PREC_MIXED && w->Vf) {
                     run_matvec<cf>(op, w->mv, w->Vf, w->mv.active, w->W);
@@ -952,6 +958,7 @@ struct ColumnSolver {
                 launch_vec_update(d, w->st, w->W, w->V, w->X, w->y, s);
         }
         HIP_CHECK(hipMemcpyAsync(hs.p, w->st, sizeof(SlotState) * B, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipMemcpyAsync(h_err.p, w->ierr, sizeof(int), hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipEventRecord(ev, s));
         pending = true;
     }
@@ -960,6 +967,7 @@ struct ColumnSolver {
         if (!pending) return;
         HIP_CHECK(hipEventSynchronize(ev));
         pending = false;
+        if (h_err.p[0]) fail(SCADYN_ERR_DEVICE, "internal: a matvec input offset fell outside the Krylov basis");
         if (op->prof.on) op->prof.collect();
         bool any_done = false;
         for (int b = 0; b < B; ++b) {

@@ -670,23 +670,27 @@ __global__ void k_store_x(KrylovDims d, const int *__restrict__ col, const cd *_
 }
 
 // matvec input offsets and activity from the slot phases: a restarting slot
-// multiplies x (block m+1 of V), an Arnoldi slot its current basis vector
+// multiplies x (block m+1 of the complex128 V), an Arnoldi slot its current
+// basis vector.  The matvec kernels read every slot's input, active or not, so
+// every offset must stay inside the input array: a restarting slot whose
+// residual is taken elsewhere (the mixed-precision solve, whose complex64 basis
+// has only m+1 blocks and no x) reads block 0.
+// nblk: n x B blocks in the input array; an offset outside it (a state the
+// solver never produces) is replaced by block 0 and reported in *err, so that
+// no matvec reads outside the array
 __global__ void k_set_inputs(KrylovDims d, const SlotState *__restrict__ st, long long *__restrict__ in_off,
-                             int *__restrict__ active, int restart_elsewhere) {
+                             int *__restrict__ active, int restart_elsewhere, int nblk, int *__restrict__ err) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= d.B) return;
     const long long nB = d.n * d.B;
-    const SlotState s = st[b];
-    long long off = 0;
     int a = 0;
-    if (s.phase == PH_RESTART) {
-        off = (long long)(d.m + 1) * nB;
-        a = restart_elsewhere ? 0 : 1;
-    } else if (s.phase == PH_ARNOLDI) {
-        off = (long long)s.i * nB;
-        a = 1;
+    long long blk = input_block(st[b], d.m, restart_elsewhere, a);
+    if (blk < 0 || blk >= nblk) {
+        blk = 0;
+        a = 0;
+        *err = 1;
     }
-    in_off[b] = off;
+    in_off[b] = blk * nB;
     active[b] = a;
 }
 
@@ -890,9 +894,9 @@ void launch_unpack_slots(const KrylovDims &d, const cd *in, const int *slot, int
     HIP_CHECK(hipGetLastError());
 }
 void launch_set_inputs(const KrylovDims &d, const SlotState *st, long long *in_off, int *active,
-                       int restart_elsewhere, hipStream_t s) {
+                       int restart_elsewhere, int nblk, int *err, hipStream_t s) {
     hipLaunchKernelGGL(k_set_inputs, dim3((d.B + 63) / 64), dim3(64), 0, s, d, st, in_off, active,
-                       restart_elsewhere);
+                       restart_elsewhere, nblk, err);
     HIP_CHECK(hipGetLastError());
 }
 void launch_retire(const KrylovDims &d, const int *col, SlotState *st, hipStream_t s) {

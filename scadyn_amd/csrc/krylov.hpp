@@ -82,11 +82,31 @@ void launch_mgsb(const KrylovDims &d, int bs, int K, cd *W, cf *Wf, const cf *V,
                  cd *H, cd *pdot, double *pnrm, double *pnav, double *nrm2, double *nav2, hipStream_t s);
 int mgsb_partials(int bs);
 
+// n x B block of the input array that slot s multiplies, and whether its
+// result is used: a restarting slot multiplies x (block m+1 of the complex128
+// V) unless its residual is taken elsewhere (then block 0, unused), an
+// Arnoldi slot its basis vector i (0 <= i < m), others block 0 (unused).  The
+// matvec reads every slot's input, so the block must exist: the complex64
+// basis of the mixed-precision solve has m+1 blocks and no x.
+__host__ __device__ inline long long input_block(const SlotState &s, int m, int restart_elsewhere, int &active) {
+    active = 0;
+    if (s.phase == PH_RESTART) {
+        active = restart_elsewhere ? 0 : 1;
+        return restart_elsewhere ? 0 : m + 1;
+    }
+    if (s.phase == PH_ARNOLDI) {
+        active = 1;
+        return s.i;
+    }
+    return 0;
+}
+
 // matvec input offsets / activity derived on the device from the slot phases;
 // restart_elsewhere = 1 leaves restarting slots inactive (their residual
-// matvec runs in the packed restart batch)
+// matvec runs in the packed restart batch).  nblk: n x B blocks of the input
+// array; *err is set (and the offset replaced by 0) for any offset outside it.
 void launch_set_inputs(const KrylovDims &d, const SlotState *st, long long *in_off, int *active,
-                       int restart_elsewhere, hipStream_t s);
+                       int restart_elsewhere, int nblk, int *err, hipStream_t s);
 // columns slot[j] (j < cap, -1 none) of X (n x B) <-> a packed n x cap batch
 void launch_pack_slots(const KrylovDims &d, const cd *X, const int *slot, int cap, cd *out, hipStream_t s);
 void launch_unpack_slots(const KrylovDims &d, const cd *in, const int *slot, int cap, cd *W, hipStream_t s);

@@ -15,6 +15,11 @@
 // 300; Krylov dimensions 20 and 50; the default number of row-sweep workgroups
 // and 7.
 //
+// It also checks that the matvec input block of every slot state lies inside
+// the input array (input_block: m+2 blocks with the complex128 basis and x, m+1
+// for the complex64 basis whose restarting slots take their residual
+// elsewhere).
+//
 // Exit status 0 when every case agrees to 1e-12 relative (H entries relative
 // to |v_k| |w|, the scale of the sum each is taken from: with a random,
 // non-orthogonal basis a coefficient can be far smaller than its terms).
@@ -205,7 +210,31 @@ static void run_case(int B, int m, long long n, int nwg, std::mt19937_64 &rng, b
                 ok ? "ok" : "FAIL");
 }
 
+// matvec input blocks of every slot state fall inside the input array: m+2
+// blocks (complex128 V with x) in the FP64 solve, m+1 (complex64 basis) when
+// restarting slots take their residual elsewhere (the mixed-precision solve)
+static void check_input_blocks() {
+    int bad = 0;
+    for (int m : {1, 2, 10, 50})
+        for (int rest = 0; rest < 2; ++rest) {
+            const int nblk = rest ? m + 1 : m + 2;
+            for (int ph : {PH_IDLE, PH_RESTART, PH_ARNOLDI, PH_DONE})
+                for (int i = 0; i < m; ++i) {
+                    SlotState s = {};
+                    s.phase = ph;
+                    s.i = i;
+                    int a = -1;
+                    const long long blk = input_block(s, m, rest, a);
+                    if (blk < 0 || blk >= nblk) ++bad;
+                    if (a != (ph == PH_ARNOLDI || (ph == PH_RESTART && !rest) ? 1 : 0)) ++bad;
+                }
+        }
+    std::printf("matvec input blocks: %s\n", bad ? "FAIL" : "ok");
+    if (bad) ++failures;
+}
+
 int main() {
+    check_input_blocks();
     std::mt19937_64 rng(11);
     const int Bs[] = {5, 64, 112, 256, 300};
     const int ms[] = {20, 50};

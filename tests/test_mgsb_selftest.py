@@ -2,7 +2,9 @@
 own per-thread and per-slot code run on the CPU by the host self-test
 (AddressSanitizer on the host build) against plain one-vector-at-a-time MGS, for
 block sizes 2, 4 and 8, every slot phase (Arnoldi at block boundaries,
-restarting, idle, done) and batch widths 5 to 300.
+restarting, idle, done) and batch widths 5 to 300; and that the matvec input
+block of every slot state lies inside the input array (complex128 basis with
+x: m+2 blocks; complex64 basis of the mixed-precision solve: m+1).
 Tolerance: 1e-12 relative on every H entry, norm and orthogonalised w."""
 import os
 import subprocess
