@@ -87,3 +87,31 @@ def test_mixed_restart_batches(case):
         assert rel(xm[r], x64[r]) < 1e-8
     op64.close()
     opm.close()
+
+
+def test_mixed_desynchronised_restarts(case):
+    """Systems of very different difficulty in one mixed-precision batch (right-
+    hand sides scaled over 6 orders and half of them smooth), so that slots
+    restart at different steps while others are in their Arnoldi steps: the
+    batch matvec then runs with restarting slots inactive, and partly filled
+    narrow FP64 restart batches and full-width ones alternate.  Every solution
+    within 1e-8 of a tight FP64 solve."""
+    from scadyn_amd import native
+    c, e, param, k = case
+    rng = np.random.default_rng(21)
+    n = 3 * e.shape[0]
+    b = rng.standard_normal((24, n)) + 1j * rng.standard_normal((24, n))
+    b[::2] = np.cumsum(b[::2], axis=1) / np.sqrt(n)   # smoother right-hand sides converge sooner
+    b *= np.logspace(-3, 3, 24)[:, None]
+    say('fp64 tight, 24 systems')
+    op64 = native.Operator(c, e, param, k, tol=1e-11, restart=20)
+    x64, it64 = op64.solve(b)
+    say('mixed, maxit 12')
+    opm = native.Operator(c, e, param, k, precision=native.PREC_MIXED, tol=1e-2, refine_tol=1e-10, maxit=12,
+                          restart=60)
+    xm, itm = opm.solve(b)
+    say(f'fp64 iterations {it64.min()}..{it64.max()}, mixed {itm.min()}..{itm.max()}')
+    for r in range(24):
+        assert rel(xm[r], x64[r]) < 1e-8
+    op64.close()
+    opm.close()
