@@ -196,7 +196,7 @@ def main():
     ap.add_argument('--refine-tol', type=float, default=1e-9, help='request loop: mixed-precision target')
     ap.add_argument('--max-active', type=int, default=8)
     ap.add_argument('--ortho', type=int, default=0, help='0 MGS (reference), 1 CGS + DGKS, 2 block MGS')
-    ap.add_argument('--mgs-block', type=int, default=0, help='MGS coefficients per pass (0 automatic, 1, 2, 4, 8)')
+    ap.add_argument('--mgs-block', type=int, default=0, help='MGS coefficients per pass (0 automatic, 1, 2, 4, 6, 8)')
     ap.add_argument('--maxit', type=int, default=50, help='Krylov dimension per cycle (reference default 50)')
     ap.add_argument('--restart', type=int, default=4, help='restart cycles (reference default 4)')
     ap.add_argument('--compare', action='store_true',

@@ -89,7 +89,7 @@ typedef struct scadyn_opts {
                             Stockham), 1 LDS Stockham on every axis                     */
     int32_t mgs_block;   /* ortho 0: basis vectors whose MGS coefficients one pass over the
                             rows evaluates (in-pass Gram correction, same recurrence):
-                            0 automatic (4), 1, 2, 4 or 8; a pass moves about
+                            0 automatic (4), 1, 2, 4, 6 or 8; a pass moves about
                             2 + 2/mgs_block vectors of HBM traffic per coefficient      */
     int32_t reserved[5];
 } scadyn_opts;

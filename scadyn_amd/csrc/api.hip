@@ -762,7 +762,7 @@ struct SolveSpec {
     double tol = 1e-5;
     int restart = 4;
     int ortho = 0;                // 0 MGS (reference), 1 CGS + DGKS, 2 block MGS
-    int mgs_block = 1;            // MGS: basis vectors per pass (1, 2, 4 or 8)
+    int mgs_block = 1;            // MGS: basis vectors per pass (1, 2, 4, 6 or 8)
     // mixed precision: every restart takes the true residual with the FP64
     // matvec and a system is done only when it is <= tol_true ||b||
     bool verify64 = false;
@@ -1381,8 +1381,9 @@ int scadyn_op_create(const scadyn_mesh *mesh, const scadyn_opts *opts, double k,
         fail(SCADYN_ERR_UNSUPPORTED, "block MGS (ortho = 2) is disabled on the device: its first run faulted and "
                                      "the cause is not yet found");
     if (o.fft_kernel != 0 && o.fft_kernel != 1) fail(SCADYN_ERR_ARG, "fft_kernel must be 0 (automatic) or 1 (Stockham)");
-    if (o.mgs_block != 0 && o.mgs_block != 1 && o.mgs_block != 2 && o.mgs_block != 4 && o.mgs_block != 8)
-        fail(SCADYN_ERR_ARG, "mgs_block must be 0 (automatic), 1, 2, 4 or 8");
+    if (o.mgs_block != 0 && o.mgs_block != 1 && o.mgs_block != 2 && o.mgs_block != 4 && o.mgs_block != 6 &&
+        o.mgs_block != 8)
+        fail(SCADYN_ERR_ARG, "mgs_block must be 0 (automatic), 1, 2, 4, 6 or 8");
     *out = nullptr;
     std::unique_ptr<scadyn_operator> op(new scadyn_operator());
     create_operator(op.get(), *mesh, o, k);

@@ -242,8 +242,9 @@ static void launch_mgsb_v(const KrylovDims &d, int bs, int K, cd *Wd, cf *Wf, co
     switch (bs) {
         case 2: launch_mgsb_t<2, VT>(d, K, Wd, Wf, V, Rhs, st, H, pdot, pnrm, pnav, nrm2, nav2, s); break;
         case 4: launch_mgsb_t<4, VT>(d, K, Wd, Wf, V, Rhs, st, H, pdot, pnrm, pnav, nrm2, nav2, s); break;
+        case 6: launch_mgsb_t<6, VT>(d, K, Wd, Wf, V, Rhs, st, H, pdot, pnrm, pnav, nrm2, nav2, s); break;
         case 8: launch_mgsb_t<8, VT>(d, K, Wd, Wf, V, Rhs, st, H, pdot, pnrm, pnav, nrm2, nav2, s); break;
-        default: fail(SCADYN_ERR_ARG, "MGS block size must be 2, 4 or 8");
+        default: fail(SCADYN_ERR_ARG, "MGS block size must be 2, 4, 6 or 8");
     }
 }
 void launch_mgsb(const KrylovDims &d, int bs, int K, cd *W, const cd *V, const cd *Rhs, const SlotState *st, cd *H,

@@ -71,7 +71,7 @@ void launch_bmgs(const KrylovDims &d, cd *W, const cd *V, const cd *Rhs, const S
                  double *pnrm, double *pnav, cd *red, double *nrm2, double *nav2, int kmax, hipStream_t s);
 int bmgs_partials();
 
-// Modified Gram-Schmidt evaluated bs (2, 4 or 8) basis vectors per pass over the
+// Modified Gram-Schmidt evaluated bs (2, 4, 6 or 8) basis vectors per pass over the
 // rows (mgsb_core.hpp): passes 0 .. ceil(K/bs) with K = max over Arnoldi slots
 // of i+1 (0 when only restarting slots).  pdot holds nwg*mgsb_partials(bs)*B.
 void launch_mgsb(const KrylovDims &d, int bs, int K, cd *W, const cd *V, const cd *Rhs, const SlotState *st, cd *H,

@@ -9,7 +9,7 @@
 // are set up in every phase (Arnoldi at columns 0, BS-1, BS, 2BS-1, 2BS, m-1;
 // restarting; idle; done) and the resulting H column, |w|^2 before and after,
 // and the orthogonalised w are compared with plain one-vector-at-a-time MGS.
-// Block sizes 2, 4, 8; basis stored as complex128 and complex64 (FP64
+// Block sizes 2, 4, 6, 8; basis stored as complex128 and complex64 (FP64
 // arithmetic on the stored values either way), w in complex128 or kept in
 // complex64 from the first pass that writes it (1e-5 then); batch widths 5, 64, 112, 256,
 // 300; Krylov dimensions 20 and 50; the default number of row-sweep workgroups
@@ -247,6 +247,7 @@ int main() {
             run_case<4, cd>(B, m, n, nwg, rng);
             run_case<8, cd>(B, m, n, nwg, rng);
             run_case<4, cd>(B, m, n, 7, rng);
+            run_case<6, cd>(B, m, n, nwg, rng);
             run_case<4, cf>(B, m, n, nwg, rng);  // complex64 basis of the mixed-precision solve
             run_case<8, cf>(B, m, n, 7, rng);
             run_case<4, cf>(B, m, n, nwg, rng, true);  // ... with complex64 w
