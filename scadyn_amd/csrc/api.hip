@@ -739,16 +739,19 @@ static int auto_batch(scadyn_operator *op, int ncols) {
     size_t per = slot_bytes(op, op->opts.maxit);
     size_t cap = (size_t)(0.6 * (double)free_b) / std::max<size_t>(per, 1);
     if (cap == 0) fail(SCADYN_ERR_MEMORY, "not enough device memory for one RHS slot");
-    // Up to 64 slots; above 64 columns, split them into equal generations of at
-    // most 128 slots when each generation fills its last 64-slot chunk to at
+    // Up to 64 slots; above 64 columns, split them into the fewest equal
+    // generations of at most 128 slots that fill their last 64-slot chunk to at
     // least 48 (the slot-chunk kernels then idle at most a quarter of a chunk)
-    // and memory allows: no nearly empty final generation of refills.
+    // and fit in memory: no nearly empty final generation of refills.
     int B = std::min(64, ncols);
-    if (ncols > 64) {
-        const int gen = (ncols + 127) / 128;
-        const int Bg = (ncols + gen - 1) / gen;
-        if ((Bg % 64 == 0 || Bg % 64 >= 48) && (size_t)Bg <= cap) B = Bg;
-    }
+    if (ncols > 64)
+        for (int gen = (ncols + 127) / 128; gen <= ncols / 48; ++gen) {
+            const int Bg = (ncols + gen - 1) / gen;
+            if ((Bg % 64 == 0 || Bg % 64 >= 48) && (size_t)Bg <= cap) {
+                B = Bg;
+                break;
+            }
+        }
     B = (int)std::min<size_t>((size_t)B, cap);
     // a memory-capped batch keeps the slot-chunk kernels: round down to a width
     // whose last 64-slot chunk holds at least 48 slots

@@ -57,6 +57,8 @@ void launch_line_reg(const LinePass &p, const C *in, C *out, const C *tw, int si
             hipLaunchKernelGGL((k_line_reg<0, C, 14>), grid, block, lds, s, in, out, p, tw, sign, fg, ci);
         else if (rm == 16)
             hipLaunchKernelGGL((k_line_reg<0, C, 16>), grid, block, lds, s, in, out, p, tw, sign, fg, ci);
+        else if (rm == 18)
+            hipLaunchKernelGGL((k_line_reg<0, C, 18>), grid, block, lds, s, in, out, p, tw, sign, fg, ci);
         else
             hipLaunchKernelGGL((k_line_reg<0, C, 20>), grid, block, lds, s, in, out, p, tw, sign, fg, ci);
     } else {
@@ -66,6 +68,8 @@ void launch_line_reg(const LinePass &p, const C *in, C *out, const C *tw, int si
             hipLaunchKernelGGL((k_line_reg<1, C, 14>), grid, block, lds, s, in, out, p, tw, sign, fg, ci);
         else if (rm == 16)
             hipLaunchKernelGGL((k_line_reg<1, C, 16>), grid, block, lds, s, in, out, p, tw, sign, fg, ci);
+        else if (rm == 18)
+            hipLaunchKernelGGL((k_line_reg<1, C, 18>), grid, block, lds, s, in, out, p, tw, sign, fg, ci);
         else
             hipLaunchKernelGGL((k_line_reg<1, C, 20>), grid, block, lds, s, in, out, p, tw, sign, fg, ci);
     }

@@ -434,11 +434,12 @@ HD void line_reg_phase(int phase, const C *__restrict__ in, C *__restrict__ out,
     }
 }
 
-// smallest compiled radix bound covering a pass (instances 12, 14, 16 and 20:
-// complex128 takes 106, 124, 139 and 174 VGPRs, i.e. 4, 4, 3 and 2 waves per SIMD)
+// smallest compiled radix bound covering a pass (instances 12, 14, 16, 18 and
+// 20: complex128 takes 106, 124, 139, 158 and 174 VGPRs, i.e. 4, 4, 3, 3 and 2
+// waves per SIMD)
 HD int reg_rmax(const LinePass &p) {
     const int r = p.rad[0] > p.rad[1] ? p.rad[0] : p.rad[1];
-    return r <= 12 ? 12 : (r <= 14 ? 14 : (r <= 16 ? 16 : 20));
+    return r <= 12 ? 12 : (r <= 14 ? 14 : (r <= 16 ? 16 : (r <= 18 ? 18 : 20)));
 }
 
 }  // namespace regfft

@@ -55,6 +55,9 @@ static void emulate(const LinePass &p, const C *in, C *out, const C *twg, int si
                 else if (rm == 16)
                     line_reg_phase<MODE, C, 16>(ph, in, out, lds.data(), tw.data(), fg, ci, p, make_ctx(p, b, t),
                                                 sign, regs[t].data());
+                else if (rm == 18)
+                    line_reg_phase<MODE, C, 18>(ph, in, out, lds.data(), tw.data(), fg, ci, p, make_ctx(p, b, t),
+                                                sign, regs[t].data());
                 else
                     line_reg_phase<MODE, C, 20>(ph, in, out, lds.data(), tw.data(), fg, ci, p, make_ctx(p, b, t),
                                                 sign, regs[t].data());

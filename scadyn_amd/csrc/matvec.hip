@@ -355,114 +355,11 @@ __global__ __launch_bounds__(256) void k_scatter_blk_w64(const C *__restrict__ x
     }
 }
 
-// Gather + near zone, one wave per non-empty cube: the 27 box nodes are loaded
-// once for all tets of the cube and each near tet's x once for all of them
-// (the rows of a cube's tets list the same columns in the same order).  Up to
-// GT tets per pass; y_c = near_c + Sc'^T g3 - k^2 S^T g_c accumulated directly.
-template <typename C, int GT>
-__global__ __launch_bounds__(256) void k_gather_cube_w64(
-    const C *__restrict__ grid, const C *__restrict__ xin, const long long *__restrict__ in_off, int B,
-    const int *__restrict__ ne_cubes, int n_ne, const int *__restrict__ cube_start, const C *__restrict__ S4,
-    int Nx, int Ny, int ncx, int ncy, int ncz, const int64_t *__restrict__ row_start,
-    const uint32_t *__restrict__ ent_blk, const cf *__restrict__ blocks, double k2,
-    const int *__restrict__ slot_active, cd *__restrict__ y) {
-    typedef typename Prec<C>::real R;
-    const int lane = threadIdx.x & 63;
-    const int nch = (B + 63) >> 6;  // last chunk may be partial: lanes >= B load slot B-1, store nothing
-    const long long nw = (long long)n_ne * nch;
-    const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const R kk = (R)k2;
-    for (long long w = (long long)blockIdx.x * 4 + wib; w < nw; w += (long long)gridDim.x * 4) {
-        const int ci = (int)(w / nch);
-        const int braw = (int)(w - (long long)ci * nch) * 64 + lane;
-        const bool valid = braw < B;
-        const int b = valid ? braw : B - 1;
-        const int c = ne_cubes[ci];
-        const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
-        const int t0 = cube_start[c], t1 = cube_start[c + 1];
-        const bool act = slot_active ? slot_active[b] != 0 : true;
-        const C *xb = xin + in_off[b] + b;
-        const size_t base = (size_t)(2 * cx) + (size_t)Nx * ((size_t)(2 * cy) + (size_t)Ny * (2 * cz));
-        for (int tp = t0; tp < t1; tp += GT) {
-            const int nt = min(GT, t1 - tp);
-            C acc[GT][3];
-#pragma unroll
-            for (int u = 0; u < GT; ++u) acc[u][0] = acc[u][1] = acc[u][2] = Prec<C>::zero();
-            for (int jz = 0; jz < 3; ++jz)
-                for (int jy = 0; jy < 3; ++jy)
-#pragma unroll
-                    for (int jx = 0; jx < 3; ++jx) {
-                        const int j = jx + 3 * jy + 9 * jz;
-                        const C *g = grid + (base + jx + (size_t)Nx * (jy + (size_t)Ny * jz)) * 4 * B + b;
-                        const C g0 = g[0], g1 = g[(size_t)B], g2 = g[(size_t)2 * B], g3 = g[(size_t)3 * B];
-#pragma unroll
-                        for (int u = 0; u < GT; ++u) {
-                            if (u >= nt) break;
-                            const C *s = S4 + ((size_t)(tp + u) * 27 + j) * 4;
-                            const C s0 = cscale(s[0], -kk);
-                            cfma(acc[u][0], s0, g0);
-                            cfma(acc[u][1], s0, g1);
-                            cfma(acc[u][2], s0, g2);
-                            cfma(acc[u][0], s[1], g3);
-                            cfma(acc[u][1], s[2], g3);
-                            cfma(acc[u][2], s[3], g3);
-                        }
-                    }
-            int64_t q = 0;
-            for (int dz = -1; dz <= 1; ++dz) {
-                const int zz = cz + dz;
-                if (zz < 0 || zz >= ncz) continue;
-                for (int dy = -1; dy <= 1; ++dy) {
-                    const int yy = cy + dy;
-                    if (yy < 0 || yy >= ncy) continue;
-                    for (int dx = -1; dx <= 1; ++dx) {
-                        const int xx = cx + dx;
-                        if (xx < 0 || xx >= ncx) continue;
-                        const int nc = xx + ncx * (yy + ncy * zz);
-                        const int c1 = cube_start[nc + 1];
-                        for (int col = cube_start[nc]; col < c1; ++col, ++q) {
-                            const C x0 = Prec<C>::from(xb[(size_t)(3 * col) * B]),
-                                    x1 = Prec<C>::from(xb[(size_t)(3 * col + 1) * B]),
-                                    x2 = Prec<C>::from(xb[(size_t)(3 * col + 2) * B]);
-#pragma unroll
-                            for (int u = 0; u < GT; ++u) {
-                                if (u >= nt) break;
-                                const uint32_t eb = ent_blk[row_start[tp + u] + q];
-                                const cf *a = blocks + (size_t)(eb & 0x7fffffffu) * 9;
-                                C m[9];
-#pragma unroll
-                                for (int i = 0; i < 9; ++i) m[i] = Prec<C>::from(a[i]);
-                                if (eb & 0x80000000u) {
-                                    cfma(acc[u][0], m[0], x0); cfma(acc[u][0], m[1], x1); cfma(acc[u][0], m[2], x2);
-                                    cfma(acc[u][1], m[3], x0); cfma(acc[u][1], m[4], x1); cfma(acc[u][1], m[5], x2);
-                                    cfma(acc[u][2], m[6], x0); cfma(acc[u][2], m[7], x1); cfma(acc[u][2], m[8], x2);
-                                } else {
-                                    cfma(acc[u][0], m[0], x0); cfma(acc[u][0], m[3], x1); cfma(acc[u][0], m[6], x2);
-                                    cfma(acc[u][1], m[1], x0); cfma(acc[u][1], m[4], x1); cfma(acc[u][1], m[7], x2);
-                                    cfma(acc[u][2], m[2], x0); cfma(acc[u][2], m[5], x1); cfma(acc[u][2], m[8], x2);
-                                }
-                            }
-                        }
-                    }
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < GT; ++u) {
-                if (u >= nt || !valid) break;
-                cd *yo = y + (size_t)(3 * (tp + u)) * B + b;
-                if (act)
-#pragma unroll
-                    for (int r = 0; r < 3; ++r) yo[(size_t)r * B] = Prec<cd>::from(acc[u][r]);
-            }
-        }
-    }
-}
-
 // Gather + near zone with the per-tet operands staged in LDS, one 64-thread
 // workgroup per (non-empty cube, 64-slot chunk); the steps are in
-// gather_core.hpp (shared with the host self-test).  Compared with scalar
-// (SMEM) operand loads this keeps many operand fetches in flight and converts
-// each near-zone block once per wave instead of once per lane.
+// gather_core.hpp (shared with the host self-test).  Staging keeps many
+// operand fetches in flight (wave-uniform operands read through scalar loads
+// would wait on each other).
 template <typename C, int NT, int GT, int VC>
 __device__ __forceinline__ void gather_pass(const gcore::GatherArgs<C> &g, const gcore::Item &it, int tp, int lane,
                                             unsigned char *smraw) {
@@ -544,17 +441,12 @@ template <typename C>
 void launch_gather_near(const MatvecArgs<C> &a, hipStream_t s) {
     if (wave_slots(a.B) && a.ne_cubes) {
         const long long nw = (long long)a.n_ne * ((a.B + 63) / 64);
-        if (a.gather_lds) {
-            const gcore::GatherArgs<C> g = {a.grid_out, a.xin,     a.in_off,    a.B,      a.ne_cubes,
-                                            a.n_ne,     a.cube_start, a.S4,      a.Nx,     a.Ny,
-                                            a.ncx,      a.ncy,       a.ncz,       a.row_start, a.ent_blk,
-                                            a.blocks,   a.k2,        a.slot_active, a.y};
-            hipLaunchKernelGGL((k_gather_lds<C, gcore::GATHER_GT, gcore::GATHER_VC>), dim3(grid_for(nw, 1)), dim3(64), 0, s, g);
-        } else {
-            hipLaunchKernelGGL((k_gather_cube_w64<C, 8>), dim3(grid_for(nw, 4)), dim3(256), 0, s, a.grid_out, a.xin,
-                               a.in_off, a.B, a.ne_cubes, a.n_ne, a.cube_start, a.S4, a.Nx, a.Ny, a.ncx, a.ncy,
-                               a.ncz, a.row_start, a.ent_blk, a.blocks, a.k2, a.slot_active, a.y);
-        }
+        const gcore::GatherArgs<C> g = {a.grid_out, a.xin,     a.in_off,    a.B,      a.ne_cubes,
+                                        a.n_ne,     a.cube_start, a.S4,      a.Nx,     a.Ny,
+                                        a.ncx,      a.ncy,       a.ncz,       a.row_start, a.ent_blk,
+                                        a.blocks,   a.k2,        a.slot_active, a.y};
+        hipLaunchKernelGGL((k_gather_lds<C, gcore::GATHER_GT, gcore::GATHER_VC>), dim3(grid_for(nw, 1)), dim3(64), 0, s,
+                           g);
         HIP_CHECK(hipGetLastError());
         return;
     }

@@ -25,7 +25,6 @@ struct MatvecArgs {
     // (NearPairs::cube_rows), which the per-cube gather relies on
     const int *ne_cubes = nullptr;
     int n_ne = 0;
-    int gather_lds = 1;  // per-cube gather with LDS-staged operands (else SMEM operands)
     double k2 = 0;
     const int *slot_active = nullptr;  // optional per-slot mask
     C *grid_in = nullptr;              // scatter output [Nz][Ny][Nx][4][B]
