@@ -11,7 +11,7 @@
 // (xcd_remap, itself checked to be a bijection), near-zone entries pointing at
 // shuffled blocks with and without the transpose flag, inactive slots (left
 // unwritten), and
-// batch widths 64, 112 (partial second chunk) and 56.
+// batch widths 64, 112 (partial second chunk), 56 and 40.
 //
 // Exit status 0 when every case agrees to 1e-13 (complex128) / 1e-5 (complex64).
 #include <algorithm>
@@ -213,7 +213,7 @@ static void check_remap() {
 int main() {
     check_remap();
     std::mt19937_64 rng(99);
-    for (int B : {64, 112, 56}) {
+    for (int B : {64, 112, 56, 40}) {
         run_case<cd>(B, rng);
         run_case<cf>(B, rng);
     }

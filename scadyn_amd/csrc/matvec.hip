@@ -414,8 +414,10 @@ __global__ __launch_bounds__(64, 3) void k_gather_lds(gcore::GatherArgs<C> g) {
     }
 }
 
-// slot-chunk (wave64-per-item) kernels when at most a quarter of the lanes idle
-static bool wave_slots(int B) { return B % 64 == 0 || B % 64 >= 48; }
+// slot-chunk (wave64-per-item) kernels when the last 64-slot chunk is at least
+// half full: even then they beat the slot-per-lane kernels, which share no
+// operand loads between the tets of a cube or the nodes of a block
+static bool wave_slots(int B) { return B % 64 == 0 || B % 64 >= 32; }
 
 static unsigned grid_for(long long tot, int bs) {
     long long g = (tot + bs - 1) / bs;
