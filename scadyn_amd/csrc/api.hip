@@ -282,6 +282,20 @@ static void make_fp32_copies(scadyn_operator *op) {
     HIP_CHECK(hipStreamSynchronize(op->s));
 }
 
+// Morton code of cube coordinates (bits of x, y, z interleaved, 21 per axis)
+static uint64_t morton3(int x, int y, int z) {
+    auto spread = [](uint64_t v) {
+        v &= 0x1fffff;
+        v = (v | v << 32) & 0x1f00000000ffffULL;
+        v = (v | v << 16) & 0x1f0000ff0000ffULL;
+        v = (v | v << 8) & 0x100f00f00f00f00fULL;
+        v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+        v = (v | v << 2) & 0x1249249249249249ULL;
+        return v;
+    };
+    return spread((uint64_t)x) | spread((uint64_t)y) << 1 | spread((uint64_t)z) << 2;
+}
+
 static void create_operator(scadyn_operator *op, const scadyn_mesh &mesh, const scadyn_opts &opts, double k) {
     if (!(k > 0.0)) fail(SCADYN_ERR_ARG, "wavenumber must be positive");
     op->opts = opts;
@@ -319,9 +333,19 @@ static void create_operator(scadyn_operator *op, const scadyn_mesh &mesh, const 
     op->d_cube_start = M.upload(g.cube_start, s);
     op->d_perm = M.upload(g.perm, s);
     if (op->np.cube_rows) {
-        std::vector<int> ne;
+        // non-empty cubes in Morton (Z-curve) order of their coordinates: the
+        // waves in flight at once then cover a compact block of cubes, whose
+        // near-zone x fits the L2 of their XCD far better than a slab of
+        // whole x rows does
+        std::vector<std::pair<uint64_t, int>> key;
         for (int c = 0; c < g.N_cubes; ++c)
-            if (g.cube_start[c + 1] > g.cube_start[c]) ne.push_back(c);
+            if (g.cube_start[c + 1] > g.cube_start[c]) {
+                const int cx = c % g.ncx, cy = (c / g.ncx) % g.ncy, cz = c / (g.ncx * g.ncy);
+                key.push_back({morton3(cx, cy, cz), c});
+            }
+        std::sort(key.begin(), key.end());
+        std::vector<int> ne;
+        for (const auto &kc : key) ne.push_back(kc.second);
         op->n_ne_cubes = (int)ne.size();
         if (!ne.empty()) op->d_ne_cubes = M.upload(ne, s);
     }

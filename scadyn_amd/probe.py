@@ -35,6 +35,7 @@ def main(argv=None):
     ap.add_argument('--restart', type=int, default=4)
     ap.add_argument('--tol', type=float, default=1e-5)
     ap.add_argument('--ortho', type=int, default=0)
+    ap.add_argument('--kernel-times', action='store_true', help='print per-kernel HIP-event averages of the matvecs')
     args = ap.parse_args(argv)
 
     if args.copy_mib:
@@ -64,6 +65,8 @@ def main(argv=None):
     op = native.Operator(c, e, param, k, batch=args.batch, precision=prec, tol=args.tol, maxit=args.maxit,
                          restart=args.restart, ortho=args.ortho)
     say(f'operator: {op.info()}')
+    if args.kernel_times:
+        op.profile(True)
     if args.matvecs:
         rng = np.random.default_rng(0)
         x = rng.standard_normal((args.batch, 3 * op.N)) + 1j * rng.standard_normal((args.batch, 3 * op.N))
@@ -72,6 +75,10 @@ def main(argv=None):
             x = op.matvec(x, precision=prec)
             x /= np.linalg.norm(x, axis=1, keepdims=True)
             say(f'matvec {i}: {time.time() - t:.3f} s (host copies included)')
+        if args.kernel_times:
+            tm = op.timings()
+            say('kernel averages (us per launch): ' +
+                str({kk: round(v['ms'] * 1e3 / v['launches']) for kk, v in tm.items() if v['launches']}))
     if args.gmres:
         t = time.time()
         _, it = op.tmatrix(args.gmres)
