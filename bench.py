@@ -127,9 +127,9 @@ def t_rel(Ta, Tb):
 # (workload, kernel, precision, batch) -> bytes
 MEASURED_TRAFFIC = {
     # profiles/pmc_traffic_matvec_ellipsoid200k_b112.md
-    ('ellipsoid200k', 'gather_near', 'fp64', 112): 28.16e9,
+    ('ellipsoid200k', 'gather_near', 'fp64', 112): 28.08e9,
     ('ellipsoid200k', 'fftz_conv', 'fp64', 112): 39.17e9,
-    ('ellipsoid200k', 'gather_near', 'mixed', 112): 15.42e9,
+    ('ellipsoid200k', 'gather_near', 'mixed', 112): 15.92e9,
     ('ellipsoid200k', 'fftz_conv', 'mixed', 112): 19.57e9,
 }
 
