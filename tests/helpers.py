@@ -18,6 +18,27 @@ def sphere_case(n_cells=4, ka=2.0, eps=2.5 + 0.01j, seed=0):
     return c, e, param, k
 
 
+def refined_sphere_case(n_cells=5, ka=2.0, eps=2.5 + 0.01j, seed=0):
+    """sphere_case with every tet of the x < 0 half split into 4 at its centroid
+    (same orientation, a quarter of the volume each): the grid step follows the
+    mean tet volume, so the refined half's cubes hold up to 17 tets, more than
+    one 8-row gather pass."""
+    c, e, _, k = sphere_case(n_cells, ka, eps, seed)
+    cent = c[e].mean(axis=1)
+    extra, tets = [], []
+    for t, tet in enumerate(e):
+        if cent[t, 0] >= 0:
+            tets.append(tet)
+            continue
+        m = c.shape[0] + len(extra)
+        extra.append(cent[t])
+        a, b, cc, d = tet
+        tets += [[m, b, cc, d], [a, m, cc, d], [a, b, m, d], [a, b, cc, m]]
+    c2 = np.vstack([c, np.array(extra)])
+    e2 = np.array(tets, dtype=e.dtype)
+    return c2, e2, np.full(e2.shape[0], complex(eps)), k
+
+
 @functools.lru_cache(maxsize=None)
 def oracle_operator(n_cells=4, ka=2.0, eps=2.5 + 0.01j, seed=0):
     from oracle import pfft_ref

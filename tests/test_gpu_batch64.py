@@ -10,7 +10,7 @@ them 24 at a time."""
 import numpy as np
 import pytest
 
-from helpers import rel, sphere_case
+from helpers import refined_sphere_case, rel, sphere_case
 
 pytestmark = pytest.mark.gpu
 
@@ -56,5 +56,37 @@ def test_solve_partly_filled_batch(case, narrow):
     say(f'iterations {ia[:8]} ... vs {ib[:8]} ...')
     assert np.array_equal(ia, ib)
     assert max(rel(xa[r], xb[r]) for r in range(70)) < 1e-12
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize('wide', [64, 112])
+def test_matvec_dense_cubes_wide_vs_narrow(wide):
+    """Cubes of up to 17 tets: the wave-per-cube gather runs several 8-row passes
+    per cube, the wave-per-node-block scatter sums more tets per node."""
+    from scadyn_amd import native
+    c, e, param, k = refined_sphere_case()
+    a = native.Operator(c, e, param, k, precision=native.PREC_MIXED)
+    b = native.Operator(c, e, param, k, precision=native.PREC_MIXED)
+    info = a.info()
+    say(f'{e.shape[0]} tets, up to {info["N_tet_cube"]} per cube, batch {wide} vs 24 at a time')
+    assert info['N_tet_cube'] > 8 and info['cube_rows'] == 1
+    rng = np.random.default_rng(14)
+    x = rng.standard_normal((wide, 3 * e.shape[0])) + 1j * rng.standard_normal((wide, 3 * e.shape[0]))
+    # This operator's output cancels strongly (rounding differences come out
+    # about 100x the unit roundoff): FP64 within 1e-12; in FP32 each kernel is
+    # held to its distance from the FP64 result, the slot-chunk one no further
+    # than 4x the slot-per-lane one.
+    ya = a.matvec(x, precision=native.PREC_FP64)
+    yb = np.concatenate([b.matvec(x[i:i + 24], precision=native.PREC_FP64) for i in range(0, wide, 24)])
+    worst = max(rel(ya[r], yb[r]) for r in range(wide))
+    say(f'FP64: worst relative difference {worst:.2e}')
+    assert worst < 1e-12
+    fa = a.matvec(x, precision=native.PREC_MIXED)
+    fb = np.concatenate([b.matvec(x[i:i + 24], precision=native.PREC_MIXED) for i in range(0, wide, 24)])
+    ea = max(rel(fa[r], ya[r]) for r in range(wide))
+    eb = max(rel(fb[r], ya[r]) for r in range(wide))
+    say(f'FP32 vs FP64: slot-chunk kernels {ea:.2e}, slot-per-lane kernels {eb:.2e}')
+    assert ea < 1e-3 and ea < 4 * eb
     a.close()
     b.close()
