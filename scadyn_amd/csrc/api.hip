@@ -334,9 +334,9 @@ static void create_operator(scadyn_operator *op, const scadyn_mesh &mesh, const 
     op->d_perm = M.upload(g.perm, s);
     if (op->np.cube_rows) {
         // non-empty cubes in Morton (Z-curve) order of their coordinates: the
-        // waves in flight at once then cover a compact block of cubes, whose
-        // near-zone x fits the L2 of their XCD far better than a slab of
-        // whole x rows does
+        // gather's waves in flight at once then cover a compact block of
+        // cubes rather than a slab of whole x rows (measured 2-3% faster at
+        // the same HBM traffic)
         std::vector<std::pair<uint64_t, int>> key;
         for (int c = 0; c < g.N_cubes; ++c)
             if (g.cube_start[c + 1] > g.cube_start[c]) {
