@@ -83,25 +83,51 @@ MGSB_HD void sweep_rows(const KrylovDims &d, int p, const WI *Win, WO *Wout, con
         h[u] = u < r.nu ? H[(size_t)b * (d.m + 1) * d.m + (size_t)(d.m + 1) * s.i + (r.ulo + u)] : cmk(0.0, 0.0);
     for (long long row = row0; row < d.n; row += rstep) {
         const size_t e = (size_t)row * d.B + b;
-        cd w = Prec<cd>::from(Win[e]);
         if (r.rst) {
-            w = csub(Rhs[e], w);
+            const cd w = csub(Rhs[e], Prec<cd>::from(Win[e]));
             Wout[e] = Prec<WO>::from(w);
             nrm += cabs2(w);
             continue;
         }
+        // With the complex64 basis all of the row's operands are fetched
+        // before any is used: the basis indices are clamped into the block (a
+        // vector beyond nu or nd is read again as vector 0 of the block and
+        // not used), so no load waits on a branch and all are in flight at
+        // once (loads issued at their use wait one by one).  The complex128
+        // basis keeps the loads at their use: fetched up front its operands
+        // take 140 VGPRs (3 waves per SIMD instead of 4), which costs more.
+        constexpr bool AHEAD = sizeof(VT) == 8;
+        const WI wraw = Win[e];
+        VT vu[BS], vd[BS];
+        if constexpr (AHEAD) {
+            if (r.nu > 0) {
+#pragma unroll
+                for (int u = 0; u < BS; ++u) vu[u] = V[(size_t)(r.ulo + (u < r.nu ? u : 0)) * nB + e];
+            }
+            if (r.nd > 0) {
+#pragma unroll
+                for (int u = 0; u < BS; ++u) vd[u] = V[(size_t)(r.dlo + (u < r.nd ? u : 0)) * nB + e];
+            }
+        }
+        cd w = Prec<cd>::from(wraw);
         if (p == 0) nav += cabs2(w);
         if (r.nu > 0) {
 #pragma unroll
             for (int u = 0; u < BS; ++u)
-                if (u < r.nu) w = csub(w, cmul(h[u], Prec<cd>::from(V[(size_t)(r.ulo + u) * nB + e])));
+                if (u < r.nu) {
+                    if constexpr (!AHEAD) vu[u] = V[(size_t)(r.ulo + u) * nB + e];
+                    w = csub(w, cmul(h[u], Prec<cd>::from(vu[u])));
+                }
             Wout[e] = Prec<WO>::from(w);
         }
         if (r.nd > 0) {
             cd v[BS];
 #pragma unroll
-            for (int u = 0; u < BS; ++u)
-                v[u] = u < r.nd ? Prec<cd>::from(V[(size_t)(r.dlo + u) * nB + e]) : cmk(0.0, 0.0);
+            for (int u = 0; u < BS; ++u) {
+                if constexpr (!AHEAD)
+                    if (u < r.nd) vd[u] = V[(size_t)(r.dlo + u) * nB + e];
+                v[u] = u < r.nd ? Prec<cd>::from(vd[u]) : cmk(0.0, 0.0);
+            }
 #pragma unroll
             for (int j = 0; j < BS; ++j) {
                 acc[j] = cadd(acc[j], cmulc(v[j], w));
