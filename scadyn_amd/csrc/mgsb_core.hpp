@@ -89,12 +89,12 @@ MGSB_HD void sweep_rows(const KrylovDims &d, int p, const WI *Win, WO *Wout, con
             nrm += cabs2(w);
             continue;
         }
-        // With the complex64 basis all of the row's operands are fetched
-        // before any is used: the basis indices are clamped into the block (a
-        // vector beyond nu or nd is read again as vector 0 of the block and
-        // not used), so no load waits on a branch and all are in flight at
-        // once (loads issued at their use wait one by one).  The complex128
-        // basis keeps the loads at their use: fetched up front its operands
+        // Operands are fetched before any is used: the basis indices are
+        // clamped into the block (a vector beyond nu or nd is read again as
+        // vector 0 of the block and not used), so no load waits on a branch
+        // and they are in flight at once (loads issued at their use wait one
+        // by one).  With the complex128 basis only the dot vectors go ahead
+        // and the update vectors are loaded at their use: all of them up front
         // take 140 VGPRs (3 waves per SIMD instead of 4), which costs more.
         constexpr bool AHEAD = sizeof(VT) == 8;
         const WI wraw = Win[e];
@@ -104,10 +104,10 @@ MGSB_HD void sweep_rows(const KrylovDims &d, int p, const WI *Win, WO *Wout, con
 #pragma unroll
                 for (int u = 0; u < BS; ++u) vu[u] = V[(size_t)(r.ulo + (u < r.nu ? u : 0)) * nB + e];
             }
-            if (r.nd > 0) {
+        }
+        if (r.nd > 0) {
 #pragma unroll
-                for (int u = 0; u < BS; ++u) vd[u] = V[(size_t)(r.dlo + (u < r.nd ? u : 0)) * nB + e];
-            }
+            for (int u = 0; u < BS; ++u) vd[u] = V[(size_t)(r.dlo + (u < r.nd ? u : 0)) * nB + e];
         }
         cd w = Prec<cd>::from(wraw);
         if (p == 0) nav += cabs2(w);
@@ -123,11 +123,7 @@ MGSB_HD void sweep_rows(const KrylovDims &d, int p, const WI *Win, WO *Wout, con
         if (r.nd > 0) {
             cd v[BS];
 #pragma unroll
-            for (int u = 0; u < BS; ++u) {
-                if constexpr (!AHEAD)
-                    if (u < r.nd) vd[u] = V[(size_t)(r.dlo + u) * nB + e];
-                v[u] = u < r.nd ? Prec<cd>::from(vd[u]) : cmk(0.0, 0.0);
-            }
+            for (int u = 0; u < BS; ++u) v[u] = u < r.nd ? Prec<cd>::from(vd[u]) : cmk(0.0, 0.0);
 #pragma unroll
             for (int j = 0; j < BS; ++j) {
                 acc[j] = cadd(acc[j], cmulc(v[j], w));
