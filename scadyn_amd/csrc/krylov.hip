@@ -593,6 +593,9 @@ __global__ __launch_bounds__(256) void k_vec_update(KrylovDims d, const SlotStat
         } else if (s.act == ACT_XUPD) {
             const cd *y = yall + (size_t)b * d.m;
             cd acc = cmk(0, 0);
+            // unrolled so that the loads of several basis vectors are in flight
+            // at once; the sum still runs in q order
+#pragma unroll 8
             for (int q = 0; q < s.len; ++q) cfma(acc, Prec<cd>::from(V[(size_t)q * nB + e]), y[q]);
             X[e] = cadd(X[e], acc);
         }
