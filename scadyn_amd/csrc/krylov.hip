@@ -579,19 +579,26 @@ template <typename VT, typename WT>
 __global__ __launch_bounds__(256) void k_vec_update(KrylovDims d, const SlotState *__restrict__ st,
                                                     const WT *__restrict__ W, VT *__restrict__ V,
                                                     cd *__restrict__ X, const cd *__restrict__ yall) {
+    // one slot per thread (its state read once), rows strided as in the sweeps
+    const RowSlotMap m = row_slot_map(d.B);
+    if (!m.ok) return;
+    const int b = m.b;
+    const SlotState s = st[b];
     const size_t nB = (size_t)d.n * d.B;
-    const long long tot = d.n * d.B;
-    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
-         e += (long long)gridDim.x * blockDim.x) {
-        const int b = (int)(e % d.B);
-        const SlotState &s = st[b];
-        if (s.act == ACT_V0 || s.act == ACT_NEXT) {
-            const double h = (s.act == ACT_V0) ? s.beta : s.hnorm;
+    const long long r0 = (long long)blockIdx.x * m.RW + m.rl, rs = (long long)gridDim.x * m.RW;
+    if (s.act == ACT_V0 || s.act == ACT_NEXT) {
+        const double h = (s.act == ACT_V0) ? s.beta : s.hnorm;
+        VT *vt = V + (size_t)s.tgt * nB;
+        for (long long row = r0; row < d.n; row += rs) {
+            const size_t e = (size_t)row * d.B + b;
             cd w = Prec<cd>::from(W[e]);
             if (h != 0.0) w = cmk(w.x / h, w.y / h);
-            V[(size_t)s.tgt * nB + e] = Prec<VT>::from(w);
-        } else if (s.act == ACT_XUPD) {
-            const cd *y = yall + (size_t)b * d.m;
+            vt[e] = Prec<VT>::from(w);
+        }
+    } else if (s.act == ACT_XUPD) {
+        const cd *y = yall + (size_t)b * d.m;
+        for (long long row = r0; row < d.n; row += rs) {
+            const size_t e = (size_t)row * d.B + b;
             cd acc = cmk(0, 0);
             // unrolled so that the loads of several basis vectors are in flight
             // at once; the sum still runs in q order
@@ -823,9 +830,9 @@ void launch_slot_update(const KrylovDims &d, SlotState *st, cd *H, cd *cs, cd *s
 template <typename VT, typename WT>
 static void launch_vec_update_t(const KrylovDims &d, const SlotState *st, const WT *W, VT *V, cd *X, const cd *y,
                                 hipStream_t s) {
-    long long tot = d.n * d.B;
-    unsigned g = (unsigned)std::min<long long>((tot + 255) / 256, 1 << 16);
-    hipLaunchKernelGGL((k_vec_update<VT, WT>), dim3(g), dim3(256), 0, s, d, st, W, V, X, y);
+    const int SB = d.B < 256 ? d.B : 256;
+    const dim3 grid((unsigned)d.nwg, (unsigned)((d.B + SB - 1) / SB));
+    hipLaunchKernelGGL((k_vec_update<VT, WT>), grid, dim3(256), 0, s, d, st, W, V, X, y);
     HIP_CHECK(hipGetLastError());
 }
 void launch_vec_update(const KrylovDims &d, const SlotState *st, const cd *W, cd *V, cd *X, const cd *y,
