@@ -1,24 +1,42 @@
-"""Benchmark: T-matrix columns/s of the JVIE solve on one MI355X.
+"""Benchmark: T-matrix columns/s of the JVIE solve on one MI355X, mixed precision
+against FP64, at a T-matrix accuracy well inside 1e-6.
 
-Workload (the headline configuration): Gaussian random ellipsoid, semi-axes
-1 : 0.8 : 0.6, log-normal radial perturbation sigma = 0.1, correlation angle
-20 deg, real spherical harmonics l <= 10 (seed 0), cube-split to 198,390 tets,
-eps = 3.0 + 0.1i, size parameter ka = 8 (a = volume-equivalent radius),
-nmax = 12, i.e. 336 right-hand sides solved as one batched launch, FP64.
-The reference's GMRES settings apply (tol 1e-5, Krylov dimension 50, 4
-restart cycles).  The same solve in mixed precision (FP32 matvecs in the
-Arnoldi steps, FP64 true residual at every restart, same iteration budget and
-tolerance) follows by default; its columns/s, the ratio to FP64 and the
-relative difference of the two T-matrices are reported beside the FP64 value.
+Workload (default): two-layer sphere, 64 cells across (centre-in, jitter 0.1 h,
+seed 0), cube-split to 824,256 tets; eps 3.5+0.2i for centroid r < 0.6 a,
+2.0+0.02i outside; ka 15; nmax 20, i.e. 880 T-matrix columns.
 
-A step is one full T-matrix solve (compute_T_matrix: VSWF right-hand sides,
-336 restarted-GMRES systems, projection GEMM) on the device-resident operator;
-the operator setup (projections, FFT(G), near zone) is built before the timed
-region and reported separately.
+Both legs solve the same columns in the same seeded random order through the
+library's continuously refilled batch (scadyn_tjob_*), each to a matched
+accuracy:
+  * FP64: the reference's GMRES (x0 = rhs, MGS, restart length 50) on the FP64
+    matvec, tol 1e-8 on |g(i+1)|/||b|| and up to 60 restart cycles, so that
+    every column converges (the reference's default of 4 cycles stops all of
+    them at the 200-iteration cap on this workload);
+  * mixed: FP32 matvec and Krylov basis (FP64 arithmetic) inside FP64
+    iterative refinement, done when the FP64 true residual is <= 1e-8 ||b||.
+The relative Frobenius difference of the two legs' T columns is reported.
 
-The CPU baseline times the oracle (numpy restatement, one core) on a bounded
-sample: its matvec on the full-size operator, scaled by the matvecs per column
-that the GPU solve needed.
+A step is K converged columns (default K = 4): the batch keeps every slot busy
+and refills a slot as soon as its column converges, so a step is the time in
+which K more columns finish.  --warmup steps run first (the first also fills
+the pipeline: no column converges before about 700 solver iterations), then
+--steps steps are timed, with a device synchronisation on both sides.  `value`
+is the mixed leg's converged columns per second over the timed steps.  The
+FP64 leg runs --fp64-warmup and --fp64-steps steps of the same kind.
+
+Per-kernel times come from the library's HIP-event profiler on the operator's
+stream over the timed steps; `roofline` names the matvec kernel with the
+largest total time in the headline precision.  The CPU baseline runs a
+multi-threaded port of the reference's matvec on the host (oracle/cpu_port.py)
+on the same operator and scales its time by the FP64 leg's matvecs per column.
+
+Other modes:
+  --workload serving | serving-small | bars16   request-loop workloads
+  --matvecs N [--batch B] [--precision P]       N batched matvecs (counter runs)
+  --copy-mib M                                  one device copy of M MiB (counter
+                                                calibration)
+  --gmres NMAX                                  iteration counts of the columns of
+                                                a smaller expansion order
 """
 import argparse
 import json
@@ -34,97 +52,19 @@ sys.path.insert(0, ROOT)
 METRIC = 'T-matrix columns/s, mixed vs FP64 pFFT-GMRES at 1e-6 rel err; p99 latency'
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFLOPS = 78.6
-ORTHO_NAMES = {0: 'mgs', 1: 'cgs+dgks', 2: 'block-mgs'}
-DTYPES = {'fp64': 'f64', 'mixed': 'f32 matvec + f64 refinement'}
+MV_CLASSES = ['scatter', 'fftx', 'ffty', 'fftz_conv', 'iffty', 'ifftx', 'gather_near']
 
 from scadyn_amd.workloads import make_workload, request_stream  # noqa: E402
 
 
-def run_requests(name, args):
-    """Request-loop workloads: throughput and per-request latency."""
-    from scadyn_amd import native
-    rng = np.random.default_rng(0)
-    reqs = request_stream(name, rng)
-    prec = native.PREC_MIXED if args.precision != 'fp64' else native.PREC_FP64
-    cols = [2 * ((r[4] + 1) ** 2 - 1) for r in reqs]
-    if name.startswith('serving'):
-        # arrival rate for ~80% utilisation from a calibration solve of the first request
-        c, e, p, k, nmax, _ = reqs[0]
-        t = time.time()
-        native.tmat_solve(c, e, p, k, nmax, precision=prec, tol=args.tol, refine_tol=args.refine_tol)
-        work0 = cols[0] * e.shape[0]
-        rate_work = work0 / (time.time() - t)          # (columns x tets) per second
-        mean_service = np.mean([cc * r[1].shape[0] for cc, r in zip(cols, reqs)]) / rate_work
-        gaps = rng.exponential(mean_service / 0.8, len(reqs))
-        arrivals = np.concatenate([[0.0], np.cumsum(gaps[1:])])
-    else:
-        arrivals = np.zeros(len(reqs))
-    print(f'[bench] {name}: {len(reqs)} requests, {sum(cols)} columns', file=sys.stderr, flush=True)
-    sv = native.Server(max_active=args.max_active)
-    t0 = time.time()
-    ids = []
-    for (c, e, p, k, nmax, _), at in zip(reqs, arrivals):
-        while time.time() - t0 < at:
-            time.sleep(0.005)
-        ids.append(sv.submit(c, e, p, k, nmax, precision=prec, tol=args.tol, refine_tol=args.refine_tol))
-    lat = []
-    for rid in ids:
-        _, it, times = sv.wait(rid)
-        lat.append(times[3] - times[0])
-    makespan = time.time() - t0
-    sv.close()
-    lat = np.array(lat)
-    return {'columns': int(sum(cols)), 'requests': len(reqs), 'makespan_s': makespan,
-            'columns_per_s': sum(cols) / makespan, 'p50_latency_s': float(np.percentile(lat, 50)),
-            'p99_latency_s': float(np.percentile(lat, 99)), 'max_active': args.max_active}
+def say(msg):
+    print(f'[bench] {msg}', file=sys.stderr, flush=True)
 
 
-def cpu_baseline(op_gpu, c, e, param, k, matvecs_per_col, budget_s=25.0):
-    """Oracle matvec on the full-size operator (operator arrays exported from the
-    device build), scaled to columns/s with the measured matvecs per column."""
-    from oracle import pfft_ref
-    t0 = time.time()
-    o = pfft_ref.Operator()
-    o.g = pfft_ref.build_geometry(c, e)
-    o.k = k
-    o.param = param
-    S = op_gpu.export('S')
-    o.S, o.Sx, o.Sy, o.Sz = S[0], S[1], S[2], S[3]
-    o.indS = np.stack([pfft_ref.box_nodes(o.g, cc)[0] for cc in o.g.tet_cube])
-    o.Fg = np.fft.fftn(pfft_ref.green_grid(o.g, k))
-    o.sp_mat = op_gpu.export('blocks')
-    o.sp_ind = op_gpu.export('blkind').astype(np.int64) - 1
-    prep = time.time() - t0
-    rng = np.random.default_rng(0)
-    x = rng.standard_normal(3 * o.g.N) + 1j * rng.standard_normal(3 * o.g.N)
-    n = 0
-    from threadpoolctl import threadpool_limits
-    with threadpool_limits(1):  # the scalar port on one core
-        t0 = time.time()
-        while True:
-            x = pfft_ref.matvec(o, x)
-            x /= np.linalg.norm(x)
-            n += 1
-            if time.time() - t0 > budget_s or n >= 4:
-                break
-        per_mv = (time.time() - t0) / n
-    cols = 1.0 / (per_mv * matvecs_per_col)
-    return {'value': cols, 'unit': 'columns/s', 'cores': 1, 'kind': 'port',
-            'sample': (f'{n} oracle matvecs (numpy, 1 thread) on the full {o.g.N}-tet operator, '
-                       f'{per_mv:.2f} s each, x {matvecs_per_col:.1f} matvecs/column measured on the '
-                       f'GPU; operator arrays exported from the device build ({prep:.0f} s prep)')}
-
-
-def t_rel(Ta, Tb):
-    """Relative Frobenius difference of two T-matrices (four nm x nm blocks each)."""
-    num = np.sqrt(sum(np.linalg.norm(a - b) ** 2 for a, b in zip(Ta, Tb)))
-    return float(num / np.sqrt(sum(np.linalg.norm(b) ** 2 for b in Tb)))
-
-
-# HBM bytes per launch of the dominant matvec kernel, from rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE runs of `python -m scadyn_amd.probe` on the same
-# workload and batch, each scaled by the copy-kernel calibration factor:
-# (workload, kernel, precision, batch) -> bytes
+# HBM bytes per launch of a matvec kernel, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE runs of `bench.py --matvecs` on the same workload and batch, each
+# scaled by the copy-kernel calibration factor (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0):
+# (workload, kernel class, precision, batch) -> bytes
 MEASURED_TRAFFIC = {
     # profiles/pmc_traffic_matvec_ellipsoid200k_b112.md
     ('ellipsoid200k', 'gather_near', 'fp64', 112): 28.08e9,
@@ -134,206 +74,272 @@ MEASURED_TRAFFIC = {
 }
 
 
-def measured_traffic(workload, kernel, precision, batch):
-    return MEASURED_TRAFFIC.get((workload, kernel, precision, batch))
+def t_rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def compare(args, c, e, param, k, nmax, desc):
-    """Mixed precision vs FP64 on the same workload at a matched accuracy.
+def kernel_table(tim):
+    out = {}
+    for n in MV_CLASSES:
+        t = tim[n]
+        if not t['launches']:
+            continue
+        avg_ms = t['ms'] / t['launches']
+        out[n] = {'avg_us': 1000.0 * avg_ms, 'launches': t['launches'],
+                  'bytes_per_launch': t['bytes'] / t['launches'],
+                  'GBps': t['bytes'] / t['launches'] / (avg_ms * 1e-3) / 1e9}
+    out['krylov_ms_total'] = tim['krylov']['ms']
+    return out
 
-    FP64: the reference's GMRES to --tol64.  Mixed: GMRES with the FP32 matvec
-    in the Arnoldi steps and the FP64 true residual at every restart, done when
-    that residual is <= --tol64 ||b|| (a cycle ends early once it has reduced
-    the residual by --inner-tol).  Reports the columns/s of both (operator
-    setup excluded), their ratio and the relative Frobenius difference of the
-    two T-matrices."""
+
+def run_leg(name, op, nmax, order, K, warmup, steps, prec_key):
+    """warmup + timed steps of K converged columns on one operator."""
     from scadyn_amd import native
+    job = native.TJob(op, nmax, order)
+    t0 = time.time()
+    done = 0
+    for w in range(warmup):
+        done = job.advance(K)
+        say(f'{name} warmup {w}: {done} columns converged, {time.time() - t0:.1f} s')
+    op.profile(True)
+    d0 = done
+    t0 = time.time()
+    for s in range(steps):
+        done = job.advance(K)
+        say(f'{name} step {s}: {done - d0} columns converged in {time.time() - t0:.1f} s')
+    elapsed = time.time() - t0
+    tim = op.timings(prec_key)
+    op.profile(False)
+    T, it = job.result()
+    job.close()
+    ncols = done - d0
+    fin = it >= 0
+    cap = op.opts.maxit * op.opts.restart
+    info = op.info()
+    r = {'columns': int(ncols), 'seconds': elapsed, 'columns_per_s': ncols / elapsed,
+         'ms_per_step': 1000.0 * elapsed / max(steps, 1), 'steps': steps, 'warmup': warmup,
+         'batch': info['batch'], 'mean_iterations': float(it[fin].mean()) if fin.any() else None,
+         'max_iterations': int(it[fin].max()) if fin.any() else None,
+         'columns_at_iteration_cap': int(np.sum(it[fin] >= cap)), 'kernels': kernel_table(tim),
+         'fft_tiles': info['fft_tiles' if prec_key == 'fp64' else 'fft_tiles_f32']}
+    say(f"{name}: {r['columns_per_s']:.3f} columns/s ({ncols} columns in {elapsed:.1f} s), batch {r['batch']}, "
+        f"mean iterations {r['mean_iterations']}")
+    return r, T, it, tim
+
+
+def roofline(tim, workload, prec, batch):
+    dom = max(MV_CLASSES, key=lambda n: tim[n]['ms'])
+    d = tim[dom]
+    avg_ms = d['ms'] / max(d['launches'], 1)
+    bpl = d['bytes'] / max(d['launches'], 1)
+    achieved = bpl / (avg_ms * 1e-3) / 1e9
+    return {'bound': 'hbm', 'kernel': dom, 'dtype': 'f32' if prec == 'mixed' else 'f64', 'achieved': achieved,
+            'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
+            'traffic': MEASURED_TRAFFIC.get((workload, dom, prec, batch)), 'avg_launch_us': 1000.0 * avg_ms,
+            'bytes_per_launch': bpl}
+
+
+def headline(args):
+    from scadyn_amd import native
+    t = time.time()
+    c, e, param, k, nmax, desc = make_workload(args.workload)
+    if args.nmax:
+        nmax = args.nmax
+        desc += f' (nmax overridden to {nmax})'
     ncol = 2 * ((nmax + 1) ** 2 - 1)
-    common = dict(batch=args.batch, ortho=args.ortho, maxit=args.maxit, restart=args.restart)
-    res = {}
-    for name, prec, tol in (('fp64', native.PREC_FP64, args.tol64), ('mixed', native.PREC_MIXED, args.inner_tol)):
-        op = native.Operator(c, e, param, k, precision=prec, tol=tol, refine_tol=args.tol64, **common)
+    order = np.random.default_rng(0).permutation(ncol)
+    say(f'workload {args.workload}: {e.shape[0]} tets, nmax {nmax}, {ncol} columns ({time.time() - t:.1f} s mesh)')
+    common = dict(maxit=args.maxit, restart=args.restart, batch=args.batch)
+    legs, Ts = {}, {}
+    cpu = None
+    for name in ('mixed', 'fp64'):
         t = time.time()
-        T, it = op.tmatrix(nmax)
-        el = time.time() - t
-        inf = op.info()
-        res[name] = {'T': T, 'seconds': el, 'columns_per_s': ncol / el, 'mean_iters': float(it.mean()),
-                     'max_iters': int(it.max()), 'max_restart_cycles': inf['outer_rounds']}
-        print(f"[bench] {name}: {el:.2f} s, {ncol / el:.3f} columns/s, mean iterations {it.mean():.1f}, "
-              f"max restart cycles {inf['outer_rounds']}", file=sys.stderr, flush=True)
+        if name == 'mixed':
+            op = native.Operator(c, e, param, k, precision=native.PREC_MIXED, tol=args.inner_tol,
+                                 refine_tol=args.tol, **common)
+            warm, steps = args.warmup, args.steps
+        else:
+            op = native.Operator(c, e, param, k, precision=native.PREC_FP64, tol=args.tol, **common)
+            warm, steps = args.fp64_warmup, args.fp64_steps
+        setup = time.time() - t
+        say(f'{name} operator set up in {setup:.1f} s')
+        r, T, it, tim = run_leg(name, op, nmax, order, args.cols_per_step, warm, steps, name)
+        r['setup_s'] = setup
+        info = op.info()
+        r['roofline'] = roofline(tim, args.workload, name, info['batch'])
+        tg = tim['tgemm']
+        if tg['launches']:
+            tf = tg['bytes'] / (tg['ms'] * 1e-3) / 1e12
+            r['assembly_gemm'] = {'kernel': 'tgemm (FP64 MFMA 16x16x4)', 'avg_us': 1000.0 * tg['ms'] / tg['launches'],
+                                  'achieved': tf, 'peak': FP64_MFMA_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                                  'frac': tf / FP64_MFMA_PEAK_TFLOPS}
+        legs[name], Ts[name] = r, (T, it)
+        if name == 'fp64' and not args.no_cpu:
+            # matvecs per column of the FP64 leg: `ite` counts the unconverged
+            # Arnoldi steps; add the converged one and one restart matvec per cycle
+            fin = it >= 0
+            mvpc = float(np.mean(it[fin] + 1 + (it[fin] // args.maxit + 1)))
+            from oracle import cpu_port
+            cpu = cpu_port.cpu_baseline(op, c, e, param, k, mvpc, budget_s=args.cpu_budget)
+            say(f"cpu baseline {cpu['value']:.4g} columns/s ({cpu['sample']})")
         op.close()
-    err = t_rel(res['mixed']['T'], res['fp64']['T'])
-    for r in res.values():
-        r.pop('T')
-    out = {'metric': METRIC, 'value': res['mixed']['columns_per_s'], 'unit': 'columns/s', 'n_gpus': 1,
-           'steps': 1, 'warmup': 0, 'ms_per_step': 1000.0 * res['mixed']['seconds'], 'higher_is_better': True,
-           'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32 matvec + f64 refinement',
-           'data': 'synthetic (seeded mesh generator)',
-           'config': {'workload': desc, 'n_tet': int(e.shape[0]), 'columns': ncol, 'nmax': nmax,
-                      'fp64': res['fp64'], 'mixed': res['mixed'],
-                      'speedup_mixed_over_fp64': res['mixed']['columns_per_s'] / res['fp64']['columns_per_s'],
-                      'T_rel_frobenius_mixed_vs_fp64': err,
-                      'gmres': {'tol64': args.tol64, 'inner_tol': args.inner_tol, 'refine_tol': args.tol64,
-                                'maxit': args.maxit, 'restart': args.restart,
-                                'ortho': ORTHO_NAMES[args.ortho]}}}
+    both = (Ts['mixed'][1] >= 0) & (Ts['fp64'][1] >= 0)
+    err = t_rel(Ts['mixed'][0][:, both], Ts['fp64'][0][:, both]) if both.any() else None
+    head = legs['mixed']
+    cfg = {'workload': desc, 'n_tet': int(e.shape[0]), 'nmax': nmax, 'columns_total': ncol,
+           'columns_per_step': args.cols_per_step, 'column_order': 'seeded random permutation (seed 0)',
+           'gmres': {'tol': args.tol, 'mixed_cycle_tol': args.inner_tol, 'maxit': args.maxit,
+                     'restart': args.restart},
+           'mixed': {kk: v for kk, v in head.items() if kk != 'roofline'},
+           'fp64': legs['fp64'],
+           'speedup_mixed_over_fp64': head['columns_per_s'] / legs['fp64']['columns_per_s'],
+           'T_rel_err_mixed_vs_fp64': err, 'T_columns_compared': int(both.sum())}
+    out = {'metric': METRIC, 'value': head['columns_per_s'], 'unit': 'columns/s', 'n_gpus': 1,
+           'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': head['ms_per_step'],
+           'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32',
+           'data': 'synthetic (seeded mesh generator)', 'config': cfg, 'roofline': head['roofline'],
+           'cpu_baseline': cpu}
     print(json.dumps(out), flush=True)
+
+
+def run_requests(name, args):
+    """Request-loop workloads: throughput and per-request latency."""
+    from scadyn_amd import native
+    rng = np.random.default_rng(0)
+    reqs = request_stream(name, rng)
+    prec = native.PREC_FP64 if args.precision == 'fp64' else native.PREC_MIXED
+    cols = [2 * ((r[4] + 1) ** 2 - 1) for r in reqs]
+    opts = dict(precision=prec, tol=args.inner_tol if prec == native.PREC_MIXED else args.tol,
+                refine_tol=args.tol, maxit=args.maxit, restart=args.restart)
+    if name.startswith('serving'):
+        # arrival rate for ~80% utilisation from a calibration solve of the first request
+        c, e, p, k, nmax, _ = reqs[0]
+        t = time.time()
+        native.tmat_solve(c, e, p, k, nmax, **opts)
+        work0 = cols[0] * e.shape[0]
+        rate_work = work0 / (time.time() - t)          # (columns x tets) per second
+        mean_service = np.mean([cc * r[1].shape[0] for cc, r in zip(cols, reqs)]) / rate_work
+        gaps = rng.exponential(mean_service / 0.8, len(reqs))
+        arrivals = np.concatenate([[0.0], np.cumsum(gaps[1:])])
+    else:
+        arrivals = np.zeros(len(reqs))
+    say(f'{name}: {len(reqs)} requests, {sum(cols)} columns')
+    sv = native.Server(max_active=args.max_active)
+    t0 = time.time()
+    ids = []
+    for (c, e, p, k, nmax, _), at in zip(reqs, arrivals):
+        while time.time() - t0 < at:
+            time.sleep(0.005)
+        ids.append(sv.submit(c, e, p, k, nmax, **opts))
+    lat = []
+    for rid in ids:
+        _, it, times = sv.wait(rid)
+        lat.append(times[3] - times[0])
+        say(f'request {rid} done: latency {lat[-1]:.2f} s')
+    makespan = time.time() - t0
+    sv.close()
+    lat = np.array(lat)
+    res = {'columns': int(sum(cols)), 'requests': len(reqs), 'makespan_s': makespan,
+           'columns_per_s': sum(cols) / makespan, 'p50_latency_s': float(np.percentile(lat, 50)),
+           'p99_latency_s': float(np.percentile(lat, 99)), 'max_active': args.max_active}
+    out = {'metric': METRIC, 'value': res['columns_per_s'], 'unit': 'columns/s', 'n_gpus': 1,
+           'steps': 1, 'warmup': 0, 'ms_per_step': 1000.0 * res['makespan_s'], 'higher_is_better': True,
+           'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64' if prec == native.PREC_FP64 else 'f32',
+           'data': 'synthetic (seeded mesh generator)', 'config': dict(workload=name, **res)}
+    print(json.dumps(out), flush=True)
+
+
+def copy_calibration(mib):
+    """One device-to-device copy of known size: the yardstick that scales the
+    FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device='cuda')
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    best = None
+    for rep in range(8):  # the first copy warms up; the best of the rest is reported
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        dst.copy_(src)
+        b.record()
+        torch.cuda.synchronize()
+        if rep:
+            best = a.elapsed_time(b) if best is None else min(best, a.elapsed_time(b))
+    say(f'copy {mib} MiB read + {mib} MiB write: best {best:.3f} ms, '
+        f'{2.0 * mib * (1 << 20) / (best * 1e-3) / 1e9:.0f} GB/s')
+
+
+def probe(args):
+    """A short, fixed amount of device work for hardware-counter runs (which
+    serialise kernels) and solver diagnostics."""
+    from scadyn_amd import native
+    c, e, param, k, nmax, desc = make_workload(args.workload)
+    prec = native.PREC_MIXED if args.precision == 'mixed' else native.PREC_FP64
+    say(f'{args.workload}: {e.shape[0]} tets ({desc})')
+    op = native.Operator(c, e, param, k, batch=args.batch or 64, precision=prec, tol=args.tol,
+                         maxit=args.maxit, restart=args.restart)
+    if args.matvecs:
+        B = op.opts.batch
+        op.profile(True)
+        rng = np.random.default_rng(0)
+        x = rng.standard_normal((B, 3 * op.N)) + 1j * rng.standard_normal((B, 3 * op.N))
+        for i in range(args.matvecs):
+            t = time.time()
+            x = op.matvec(x, precision=prec)
+            x /= np.linalg.norm(x, axis=1, keepdims=True)
+            say(f'matvec {i}: {time.time() - t:.3f} s (host copies included)')
+        tm = op.timings(args.precision)
+        say('kernel averages (us per launch): ' +
+            str({kk: round(v['ms'] * 1e3 / v['launches']) for kk, v in tm.items() if v['launches']}))
+    if args.gmres:
+        t = time.time()
+        _, it = op.tmatrix(args.gmres)
+        cap = args.maxit * args.restart
+        say(f'T-matrix nmax {args.gmres}: {len(it)} columns in {time.time() - t:.2f} s; '
+            f'maxit {args.maxit} restart {args.restart}: iterations {it.tolist()}; '
+            f'{int(np.sum(it >= cap))} columns at the cap {cap}')
+    op.close()
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    # a step is one whole 336-column T-matrix solve (tens of seconds); kernels are
-    # compiled ahead of time, so no warm-up step is needed for steady timings
-    ap.add_argument('--steps', type=int, default=1)
-    ap.add_argument('--warmup', type=int, default=0)
-    ap.add_argument('--workload', default='ellipsoid200k')
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=5)
+    ap.add_argument('--workload', default='layered824k')
+    ap.add_argument('--cols-per-step', type=int, default=4, help='converged columns per step')
+    ap.add_argument('--fp64-steps', type=int, default=3)
+    ap.add_argument('--fp64-warmup', type=int, default=1)
     ap.add_argument('--batch', type=int, default=0)
-    ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--precision', default='both', choices=['fp64', 'mixed', 'both'],
-                    help='both: the FP64 solve (the reported value) and the mixed-precision one, with '
-                         'their ratio and the relative difference of the two T-matrices')
-    ap.add_argument('--tol', type=float, default=1e-5)
-    ap.add_argument('--refine-tol', type=float, default=1e-9, help='request loop: mixed-precision target')
-    ap.add_argument('--max-active', type=int, default=8)
-    ap.add_argument('--ortho', type=int, default=0, help='0 MGS (reference), 1 CGS + DGKS, 2 block MGS')
-    ap.add_argument('--mgs-block', type=int, default=0, help='MGS coefficients per pass (0 automatic, 1, 2, 4, 6, 8)')
-    ap.add_argument('--maxit', type=int, default=50, help='Krylov dimension per cycle (reference default 50)')
-    ap.add_argument('--restart', type=int, default=4, help='restart cycles (reference default 4)')
-    ap.add_argument('--compare', action='store_true',
-                    help='mixed precision vs FP64 at a matched accuracy: both T-matrices, their '
-                         'relative difference and the columns/s of each')
-    ap.add_argument('--tol64', type=float, default=1e-8,
-                    help='--compare: FP64 GMRES tolerance, and the true-residual target of the mixed solve')
-    ap.add_argument('--inner-tol', type=float, default=1e-4,
-                    help='--compare: residual reduction that ends a mixed-precision restart cycle early')
     ap.add_argument('--nmax', type=int, default=0, help='override the workload expansion order')
+    ap.add_argument('--tol', type=float, default=1e-8,
+                    help='FP64 GMRES tolerance, and the FP64 true-residual target of the mixed solve')
+    ap.add_argument('--inner-tol', type=float, default=1e-4,
+                    help='mixed: residual reduction that ends a restart cycle early')
+    ap.add_argument('--maxit', type=int, default=50, help='Krylov dimension per cycle (reference default 50)')
+    ap.add_argument('--restart', type=int, default=60, help='restart cycles (enough for every column to converge)')
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--cpu-budget', type=float, default=25.0, help='seconds of CPU-baseline work')
+    ap.add_argument('--precision', default='mixed', choices=['fp64', 'mixed'],
+                    help='request-loop and probe modes')
+    ap.add_argument('--max-active', type=int, default=8)
+    ap.add_argument('--matvecs', type=int, default=0)
+    ap.add_argument('--copy-mib', type=int, default=0)
+    ap.add_argument('--gmres', type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU path is not built.',
               file=sys.stderr)
         sys.exit(2)
-    from scadyn_amd import native
-    if args.workload in ('bars16', 'serving', 'serving-small'):
-        res = run_requests(args.workload, args)
-        out = {'metric': METRIC, 'value': res['columns_per_s'], 'unit': 'columns/s', 'n_gpus': 1,
-               'steps': 1, 'warmup': 0, 'ms_per_step': 1000.0 * res['makespan_s'], 'higher_is_better': True,
-               'scaling': 'weak', 'vs_baseline': None,
-               'dtype': DTYPES['fp64' if args.precision == 'fp64' else 'mixed'],
-               'data': 'synthetic (seeded mesh generator)', 'config': dict(workload=args.workload, **res)}
-        print(json.dumps(out), flush=True)
-        return
-    c, e, param, k, nmax, desc = make_workload(args.workload)
-    if args.nmax:
-        nmax = args.nmax
-        desc += f' (nmax overridden to {nmax})'
-    print(f'[bench] workload {args.workload}: {e.shape[0]} tets, nmax {nmax}', file=sys.stderr, flush=True)
-    if args.compare:
-        compare(args, c, e, param, k, nmax, desc)
-        return
-    legs = ['fp64', 'mixed'] if args.precision == 'both' else [args.precision]
-    runs, cpu = {}, None
-    for name in legs:
-        r = runs[name] = measure(args, name, c, e, param, k, nmax)
-        if name == legs[0] and not args.no_cpu:
-            cpu = cpu_baseline(r['op'], c, e, param, k, r['matvecs_per_column'])
-            print(f"[bench] cpu baseline {cpu['value']:.4g} columns/s", file=sys.stderr, flush=True)
-        r.pop('op').close()  # one operator resident at a time
-    head = runs[legs[0]]
-    ncol = head['columns']
-    cfg = {'workload': desc, 'n_tet': int(e.shape[0]), 'columns': ncol, 'nmax': nmax,
-           'grid': head['grid'], 'fft': head['fft'], 'near_blocks': head['near_blocks'],
-           'gmres': {'tol': args.tol, 'maxit': args.maxit, 'restart': args.restart,
-                     'ortho': ORTHO_NAMES[args.ortho], 'mgs_block': args.mgs_block}}
-    for name, r in runs.items():
-        cfg[name] = {kk: v for kk, v in r.items() if kk not in ('T', 'roofline', 'kernels', 'assembly_gemm')}
-        cfg[name]['kernels'] = r['kernels']
-    if len(runs) == 2:
-        cfg['speedup_mixed_over_fp64'] = runs['mixed']['columns_per_s'] / runs['fp64']['columns_per_s']
-        cfg['T_rel_frobenius_mixed_vs_fp64'] = t_rel(runs['mixed']['T'], runs['fp64']['T'])
-    out = {
-        'metric': METRIC, 'value': head['columns_per_s'], 'unit': 'columns/s', 'n_gpus': 1,
-        'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': head['ms_per_step'],
-        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
-        'dtype': DTYPES[legs[0]], 'data': 'synthetic (seeded mesh generator)', 'config': cfg,
-        'roofline': head['roofline'], 'cpu_baseline': cpu, 'assembly_gemm': head['assembly_gemm'],
-    }
-    print(json.dumps(out), flush=True)
-
-
-def measure(args, name, c, e, param, k, nmax):
-    """Operator setup, then --warmup untimed and --steps timed T-matrix solves in
-    one precision; per-kernel timings from the library's HIP-event profiler."""
-    from scadyn_amd import native
-    t0 = time.time()
-    prec = native.PREC_MIXED if name == 'mixed' else native.PREC_FP64
-    # mixed: a restart cycle ends once the Arnoldi estimate has fallen by --tol,
-    # and a system is done when its FP64 true residual is <= --tol ||b||, the
-    # criterion of the FP64 GMRES
-    op = native.Operator(c, e, param, k, batch=args.batch, precision=prec, tol=args.tol,
-                         refine_tol=args.tol, ortho=args.ortho, mgs_block=args.mgs_block, maxit=args.maxit,
-                         restart=args.restart)
-    setup_s = time.time() - t0
-    ncol = 2 * ((nmax + 1) ** 2 - 1)
-    for w in range(args.warmup):
-        t = time.time()
-        _, it = op.tmatrix(nmax)
-        print(f'[bench] {name} warmup {w}: {time.time() - t:.2f} s, mean iters {it.mean():.1f}', file=sys.stderr,
-              flush=True)
-    op.profile(True)
-    t0 = time.time()
-    iters = []
-    for s in range(args.steps):
-        T, it = op.tmatrix(nmax)
-        iters.append(it)
-        print(f'[bench] {name} step {s}: {time.time() - t0:.2f} s cumulative', file=sys.stderr, flush=True)
-    elapsed = time.time() - t0
-    tim = op.timings()
-    op.profile(False)
-    info = op.info()
-    it = np.concatenate(iters)
-    # matvecs per column: `ite` counts unconverged inner iterations; add the
-    # converged one and one restart matvec per cycle
-    mv_per_col = float(np.mean(it + 1 + (it // op.opts.maxit + 1)))
-    mv_classes = ['scatter', 'fftx', 'ffty', 'fftz_conv', 'iffty', 'ifftx', 'gather_near']
-    dom = max(mv_classes, key=lambda n: tim[n]['ms'])
-    d = tim[dom]
-    avg_ms = d['ms'] / max(d['launches'], 1)
-    bytes_per_launch = d['bytes'] / max(d['launches'], 1)
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    per_kernel = {n: {'avg_us': 1000.0 * tim[n]['ms'] / max(tim[n]['launches'], 1),
-                      'launches': tim[n]['launches'],
-                      'GBps': (tim[n]['bytes'] / max(tim[n]['launches'], 1))
-                      / max(1e-12, tim[n]['ms'] / max(tim[n]['launches'], 1) * 1e-3) / 1e9}
-                  for n in mv_classes}
-    per_kernel['krylov'] = {'ms_total': tim['krylov']['ms']}
-    tg = tim['tgemm']
-    gemm = None
-    if tg['launches']:
-        tflops = tg['bytes'] / (tg['ms'] * 1e-3) / 1e12
-        gemm = {'kernel': 'tgemm (FP64 MFMA 16x16x4)', 'avg_us': 1000.0 * tg['ms'] / tg['launches'],
-                'achieved': tflops, 'peak': FP64_MFMA_PEAK_TFLOPS, 'unit': 'TFLOP/s',
-                'frac': tflops / FP64_MFMA_PEAK_TFLOPS}
-        per_kernel['tgemm'] = gemm
-    print(f'[bench] {name} kernels ' + json.dumps(per_kernel), file=sys.stderr, flush=True)
-    value = ncol * args.steps / elapsed
-    print(f"[bench] {name}: {value:.3f} columns/s, mean iterations {it.mean():.1f}, batch {info['batch']}, "
-          f"fft tiles {info['fft_tiles']} / {info['fft_tiles_f32']}", file=sys.stderr, flush=True)
-    dt = 'f32' if name == 'mixed' else 'f64'
-    return {
-        'op': op, 'T': T, 'columns': ncol, 'columns_per_s': value, 'ms_per_step': 1000.0 * elapsed / args.steps,
-        'seconds': elapsed, 'setup_s': setup_s, 'columns_per_s_with_setup': ncol * args.steps / (
-            elapsed + setup_s * args.steps),
-        'grid': [info['Nx'], info['Ny'], info['Nz']], 'fft': [info['Lx'], info['Ly'], info['Lz']],
-        'near_blocks': info['n_blocks'], 'batch': info['batch'], 'fft_tiles': info['fft_tiles'],
-        'fft_tiles_f32': info['fft_tiles_f32'], 'mean_gmres_iters': float(it.mean()),
-        'matvecs_per_column': mv_per_col, 'max_restart_cycles': info['outer_rounds'],
-        'columns_at_iteration_cap': int(np.sum(it >= args.maxit * args.restart)),
-        'kernels': per_kernel, 'assembly_gemm': gemm,
-        'roofline': {'bound': 'hbm', 'kernel': dom, 'dtype': dt, 'achieved': achieved, 'peak': HBM_PEAK_GBS,
-                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
-                     'traffic': measured_traffic(args.workload, dom, name, info['batch']),
-                     'avg_launch_us': avg_ms * 1000.0, 'bytes_per_launch': bytes_per_launch},
-    }
+    if args.copy_mib:
+        copy_calibration(args.copy_mib)
+    elif args.matvecs or args.gmres:
+        probe(args)
+    elif args.workload in ('bars16', 'serving', 'serving-small'):
+        run_requests(args.workload, args)
+    else:
+        headline(args)
 
 
 if __name__ == '__main__':

@@ -171,6 +171,29 @@ int scadyn_op_tmatrix(scadyn_operator *op, int32_t nmax,
                       double *Taa, double *Tab, double *Tba, double *Tbb,
                       int32_t *iters);
 
+/* Selected columns of the T-matrix: only the systems of columns cols[0..ncols-1]
+ * (0-based, M-waves 0..nm-1 then N-waves) are solved -- the body of
+ * compute_T_matrix's column loop (src/T_matrix.f90:254-266) for those columns --
+ * and T(:, cols[j]) over all 2nm rows, scaled by i k^3 (:270-275), is written
+ * column-major into Tcols (2nm x ncols).  iters[ncols] (optional). */
+int scadyn_op_tmatrix_cols(scadyn_operator *op, int32_t nmax, int32_t ncols, const int32_t *cols,
+                           double *Tcols, int32_t *iters);
+
+/* The same as a resumable job, for callers that take results as columns
+ * converge (throughput measurement, streaming): the columns enter the
+ * continuously refilled batch in the order given.  No counterpart in the
+ * reference, whose column loop (src/T_matrix.f90:254-266) runs to the end.
+ * advance runs solver steps until at least `want` more systems have finished
+ * (or all have) and reports how many have finished in total; result writes
+ * T(:, cols[j]) as scadyn_op_tmatrix_cols does for finished columns and zero
+ * with iters[j] = -1 for the rest.  The operator must outlive the job. */
+typedef struct scadyn_tjob scadyn_tjob;
+int scadyn_tjob_create(scadyn_operator *op, int32_t nmax, int32_t ncols, const int32_t *cols,
+                       scadyn_tjob **out);
+int scadyn_tjob_advance(scadyn_tjob *job, int32_t want, int32_t *ndone);
+int scadyn_tjob_result(scadyn_tjob *job, double *Tcols, int32_t *iters);
+void scadyn_tjob_destroy(scadyn_tjob *job);
+
 /* Kernel timing: when enabled, every launch of the hot kernels is bracketed
  * by HIP events on the operator's stream.  timings[i] = {total ms, launches,
  * algorithmic bytes} for kernel class i (see SCADYN_KCLASS_*); for the T-matrix
@@ -188,6 +211,11 @@ int scadyn_op_tmatrix(scadyn_operator *op, int32_t nmax,
 #define SCADYN_KCLASS_COUNT    9
 int scadyn_op_profile(scadyn_operator *op, int32_t enable);
 int scadyn_op_timings(scadyn_operator *op, double *timings /* [COUNT][3] */);
+/* The same with the matvec classes (SCADYN_KCLASS_SCATTER .. GATHER) restricted
+ * to the launches of one matvec precision (SCADYN_PREC_FP64: complex128 grids,
+ * SCADYN_PREC_MIXED: complex64); the Krylov and GEMM classes are FP64 and
+ * reported whole. */
+int scadyn_op_timings_prec(scadyn_operator *op, int32_t precision, double *timings /* [COUNT][3] */);
 
 /* ---- request loop (serving) ------------------------------------------ */
 
@@ -203,7 +231,12 @@ typedef struct scadyn_server_opts {
     int32_t device;       /* HIP device ordinal                                   */
     int32_t max_active;   /* requests resident on the device at once (0: 8)       */
     int64_t hbm_budget;   /* device bytes the server may hold (0: 80% of free)    */
-    int32_t reserved[8];
+    int32_t host_only;    /* nonzero: no device is used; requests are validated,
+                             admitted by footprint and retired after a stand-in
+                             "solve" that computes nothing (T blocks zero, iteration
+                             counts = the request id); tests the loop's admission
+                             and completion logic on a host without a GPU        */
+    int32_t reserved[7];
 } scadyn_server_opts;
 
 #define SCADYN_REQ_QUEUED  0

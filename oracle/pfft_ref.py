@@ -549,6 +549,64 @@ def matvec(op, x):
     return near_matvec(op, x) + Yaim
 
 
+def exact_pair_terms(coord, etopol, k, chunk=2048):
+    """V_V and dV_dV integrals of every tet pair (T <= B), no grid terms.
+
+    Returns (T, B, VV (P,), dVdV (P,3,3), vol (N,)).  These are the exact
+    interactions that the reference evaluates for near pairs
+    (src/precorrection.f90:227-246 before the precorrection is subtracted);
+    here every pair is integrated, near or far.
+    """
+    tets = coord[etopol]
+    if (tetra_det(tets) <= 0).any():
+        raise ValueError('detJ <= 0')
+    N = tets.shape[0]
+    T, B = np.triu_indices(N)
+    VV = np.zeros(T.size, complex)
+    dVdV = np.zeros((T.size, 3, 3), complex)
+    for s0 in range(0, T.size, chunk):
+        sl = slice(s0, min(T.size, s0 + chunk))
+        VV[sl] = integrate_V_V_G(tets[T[sl]], tets[B[sl]], k)
+        dVdV[sl] = integrate_dV_dV_G(tets[T[sl]], tets[B[sl]], k)
+    return T, B, VV, dVdV, tetra_volume(tets)
+
+
+def exact_matrix(coord, etopol, param, k, terms=None, k2_sign=-1.0, self_term=True):
+    """Dense Galerkin matrix of the order-0 JVIE (3N x 3N, rows 3t+c), every
+    pair integrated exactly:
+
+        A_TB = delta_TB/(eps_T - 1) I + (-k^2 V_V I + dV_dV) / sqrt(vol_T vol_B)
+
+    the block formula of src/precorrection.f90:227-246 without the grid
+    terms, i.e. the operator that the pFFT matvec (grid part + precorrected
+    near zone) approximates.  A is complex-symmetric: A_BT = A_TB^T.
+
+    k2_sign and self_term exist so that tests can plant a wrong sign or drop
+    the self term and show that the Mie comparison catches it."""
+    T, B, VV, dVdV, vol = terms if terms is not None else exact_pair_terms(coord, etopol, k)
+    param = np.asarray(param, complex)
+    N = vol.size
+    eye = np.eye(3)
+    blk = (k2_sign * k ** 2 * VV[:, None, None] * eye[None] + dVdV) / np.sqrt(vol[T] * vol[B])[:, None, None]
+    if self_term:
+        d = T == B
+        blk[d] += eye[None] * (1.0 / (param[T[d]] - 1.0))[:, None, None]
+    A = np.zeros((N, 3, N, 3), complex)
+    A[T, :, B, :] = blk
+    off = T != B
+    A[B[off], :, T[off], :] = blk[off].transpose(0, 2, 1)
+    return A.reshape(3 * N, 3 * N)
+
+
+def tmatrix_dense(A, mat, k):
+    """T blocks of a dense operator: i k^3 mat^H A^-1 mat split as compute_T_matrix
+    (src/T_matrix.f90:254-275), solved directly."""
+    X = np.linalg.solve(A, mat)
+    Tm = 1j * k ** 3 * (np.conj(mat).T @ X)
+    nm = Tm.shape[0] // 2
+    return Tm[:nm, :nm], Tm[:nm, nm:], Tm[nm:, :nm], Tm[nm:, nm:]
+
+
 def dense_matrix(op):
     """Dense A by unit vectors (tiny meshes only)."""
     n = 3 * op.g.N

@@ -76,7 +76,8 @@ struct Profiler {
     };
     std::vector<Pending> pend;
     std::vector<hipEvent_t> pool;
-    double ms[SCADYN_KCLASS_COUNT] = {0}, cnt[SCADYN_KCLASS_COUNT] = {0}, byt[SCADYN_KCLASS_COUNT] = {0};
+    // per kernel class, FP64 launches in [0, COUNT), complex64 ones in [COUNT, 2 COUNT)
+    double ms[2 * SCADYN_KCLASS_COUNT] = {0}, cnt[2 * SCADYN_KCLASS_COUNT] = {0}, byt[2 * SCADYN_KCLASS_COUNT] = {0};
     hipEvent_t ev() {
         if (!pool.empty()) {
             hipEvent_t e = pool.back();
@@ -611,18 +612,19 @@ static void run_matvec(scadyn_operator *op, MatvecSpace &w, const C *xin, const 
     a.grid_out = g0;
     a.y = y;
     const double c = sizeof(C);
+    const int po = pr * SCADYN_KCLASS_COUNT;  // profiler bucket of this precision
     const double nB = 4.0 * w.B;
     const double Lx = op->L[0], Ly = op->L[1], Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
     {
-        ProfScope ps(op, SCADYN_KCLASS_SCATTER, c * (Nx * Ny * Nz * nB + 27.0 * 4 * g.N) + c * 3 * g.N * w.B);
+        ProfScope ps(op, SCADYN_KCLASS_SCATTER + po, c * (Nx * Ny * Nz * nB + 27.0 * 4 * g.N) + c * 3 * g.N * w.B);
         launch_scatter<C>(a, op->s);
     }
     {
-        ProfScope ps(op, SCADYN_KCLASS_FFTX, c * nB * (Nx * Ny * Nz + Lx * Ny * Nz));
+        ProfScope ps(op, SCADYN_KCLASS_FFTX + po, c * nB * (Nx * Ny * Nz + Lx * Ny * Nz));
         launch_line_fft<C>(w.pass[pr][0], g0, g1, tw, -1, op->s);
     }
     {
-        ProfScope ps(op, SCADYN_KCLASS_FFTY, c * nB * (Lx * Ny * Nz + Lx * Ly * Nz));
+        ProfScope ps(op, SCADYN_KCLASS_FFTY + po, c * nB * (Lx * Ny * Nz + Lx * Ly * Nz));
         launch_line_fft<C>(w.pass[pr][1], g1, g0, tw + op->L[0], -1, op->s);
     }
     {
@@ -633,20 +635,20 @@ static void run_matvec(scadyn_operator *op, MatvecSpace &w, const C *xin, const 
         ci.Hx = op->Hh[0];
         ci.Hy = op->Hh[1];
         ci.nb = 4LL * w.B;
-        ProfScope ps(op, SCADYN_KCLASS_FFTZCONV,
+        ProfScope ps(op, SCADYN_KCLASS_FFTZCONV + po,
                      c * (nB * 2.0 * Lx * Ly * Nz + (double)op->Hh[0] * op->Hh[1] * op->Hh[2]));
         launch_line_conv<C>(w.pass[pr][2], g0, g0, tw + op->L[0] + op->L[1], OpData<C>::fg(op), ci, op->s);
     }
     {
-        ProfScope ps(op, SCADYN_KCLASS_IFFTY, c * nB * (Lx * Ly * Nz + Lx * Ny * Nz));
+        ProfScope ps(op, SCADYN_KCLASS_IFFTY + po, c * nB * (Lx * Ly * Nz + Lx * Ny * Nz));
         launch_line_fft<C>(w.pass[pr][3], g0, g1, tw + op->L[0], +1, op->s);
     }
     {
-        ProfScope ps(op, SCADYN_KCLASS_IFFTX, c * nB * (Lx * Ny * Nz + Nx * Ny * Nz));
+        ProfScope ps(op, SCADYN_KCLASS_IFFTX + po, c * nB * (Lx * Ny * Nz + Nx * Ny * Nz));
         launch_line_fft<C>(w.pass[pr][4], g1, g0, tw, +1, op->s);
     }
     {
-        ProfScope ps(op, SCADYN_KCLASS_GATHER,
+        ProfScope ps(op, SCADYN_KCLASS_GATHER + po,
                      c * (Nx * Ny * Nz * nB + 27.0 * 4 * g.N) + (c + 16.0) * 3 * g.N * w.B + 80.0 * op->nblocks);
         launch_gather_near<C>(a, op->s);
     }
@@ -833,6 +835,7 @@ struct ColumnSolver {
     Pinned<SlotState> hs;
     Pinned<int> h_load, h_ret, h_rs, h_ract, h_err;
     std::vector<int> slot_col;
+    std::vector<char> col_fin;  // 1 once column c (index into src/dst) has been retired
     int next = 0, nfail = 0, singular = 0, max_cycles = 0;
     long long steps = 0, max_steps = 0;
     hipEvent_t ev = nullptr;
@@ -872,8 +875,10 @@ struct ColumnSolver {
         }
         std::memset(hs.p, 0, sizeof(SlotState) * B);
         slot_col.assign(B, -1);
+        col_fin.assign((size_t)std::max<long long>(ld, 1), 0);
         const size_t nB = (size_t)w->n * B;
         HIP_CHECK(hipMemsetAsync(w->st, 0, sizeof(SlotState) * B, op->s));
+        HIP_CHECK(hipMemsetAsync(w->ierr, 0, sizeof(int), op->s));  // the workspace may be reused
         if (w->Vf) {
             HIP_CHECK(hipMemsetAsync(w->Vf, 0, sizeof(cf) * (size_t)(w->m + 1) * nB, op->s));
             HIP_CHECK(hipMemsetAsync(w->X, 0, sizeof(cd) * nB, op->s));
@@ -1005,6 +1010,7 @@ struct ColumnSolver {
                 max_cycles = std::max(max_cycles, hs.p[b].cycle + 1);
                 if (hs.p[b].conv == 0) nfail++;
                 if (hs.p[b].conv < 0) singular++;
+                col_fin[slot_col[b]] = 1;
                 slot_col[b] = -1;
                 hs.p[b].phase = PH_IDLE;
                 any_done = true;
@@ -1083,18 +1089,34 @@ static void solve_all(scadyn_operator *op, const cd *Bm, long long n, int ld, cd
 // host loop can interleave the steps of several jobs on different streams.
 // All 2nm columns run as one continuously refilled batch (solve_spec gives
 // the precision mode).
+// src[row*ns + j] = mat[row*ncol + sel[j]]: the right-hand sides of a column subset
+__global__ void k_select_cols(const cd *__restrict__ mat, long long n, int ncol, const int *__restrict__ sel, int ns,
+                              cd *__restrict__ src) {
+    const long long tot = n * ns;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
+         e += (long long)gridDim.x * blockDim.x) {
+        const long long row = e / ns;
+        src[e] = mat[row * ncol + sel[e % ns]];
+    }
+}
+
+// One T-matrix solve (compute_T_matrix, src/T_matrix.f90:240-277) as a
+// resumable job: launch()/finish() advance its column solver by one step, so a
+// host loop can interleave the steps of several jobs on different streams.
+// All 2nm columns (or a given subset of them) run as one continuously
+// refilled batch (solve_spec gives the precision mode).
 struct TmatJob {
     scadyn_operator *op = nullptr;
-    int nmax = 0, nm = 0, ncol = 0;
+    int nmax = 0, nm = 0, ncol = 0, ns = 0;  // ns: columns solved (all, or the subset)
     long long n = 0;
     DevMem tmp;
-    cd *mat = nullptr, *X = nullptr;
+    cd *mat = nullptr, *src = nullptr, *X = nullptr;
     std::vector<int32_t> iters;
     int rounds = 0, nfail = 0;
     bool finished = false;
     std::unique_ptr<ColumnSolver> cs;
 
-    void start(scadyn_operator *o, int nmax_) {
+    void start(scadyn_operator *o, int nmax_, const std::vector<int> *subset = nullptr) {
         op = o;
         nmax = nmax_;
         if (nmax < 1) fail(SCADYN_ERR_ARG, "nmax must be >= 1");
@@ -1104,12 +1126,27 @@ struct TmatJob {
         hipStream_t s = op->s;
         mat = tmp.alloc<cd>((size_t)n * ncol);
         launch_vswf(op->d_tcoord, op->d_vol, op->g.N, nmax, op->k, mat, ncol, s);
-        X = tmp.alloc<cd>((size_t)n * ncol);
-        iters.assign(ncol, 0);
-        std::vector<int> all(ncol);
-        for (int c = 0; c < ncol; ++c) all[c] = c;
+        src = mat;
+        ns = ncol;
+        if (subset) {
+            ns = (int)subset->size();
+            if (ns < 1) fail(SCADYN_ERR_ARG, "empty column subset");
+            for (int c : *subset)
+                if (c < 0 || c >= ncol) fail(SCADYN_ERR_ARG, "column index outside 0 .. 2 nm - 1");
+            int *d_sel = tmp.upload(*subset, s);
+            src = tmp.alloc<cd>((size_t)n * ns);
+            const long long tot = n * ns;
+            hipLaunchKernelGGL(k_select_cols, dim3((unsigned)std::min<long long>((tot + 255) / 256, 1 << 16)),
+                               dim3(256), 0, s, mat, n, ncol, d_sel, ns, src);
+            HIP_CHECK(hipGetLastError());
+        }
+        X = tmp.alloc<cd>((size_t)n * ns);
+        HIP_CHECK(hipMemsetAsync(X, 0, sizeof(cd) * (size_t)n * ns, s));  // columns not yet solved read as zero
+        iters.assign(ns, 0);
+        std::vector<int> all(ns);
+        for (int c = 0; c < ns; ++c) all[c] = c;
         cs.reset(new ColumnSolver());
-        cs->init(op, mat, ncol, all, X, iters.data(), solve_spec(op));
+        cs->init(op, src, ns, all, X, iters.data(), solve_spec(op));
     }
 
     void launch() {
@@ -1127,36 +1164,58 @@ struct TmatJob {
         finished = true;
     }
 
-    // T = i k^3 mat^H X split into the four nm x nm column-major blocks
-    void result(double *Taa, double *Tab, double *Tba, double *Tbb, int32_t *it_out) {
+    // i k^3 mat^H X: ncol x ns, row-major (i, j), on the host
+    std::vector<cd> scaled_T() {
         hipStream_t s = op->s;
-        const size_t work = std::max<size_t>((size_t)ncol * ncol * 64, 1 << 20);
+        const size_t work = std::max<size_t>((size_t)ncol * ns * 64, 1 << 20);
         DevMem t2;
         cd *wk = t2.alloc<cd>(work);
-        cd *T = t2.alloc<cd>((size_t)ncol * ncol);
+        cd *T = t2.alloc<cd>((size_t)ncol * ns);
         {
-            ProfScope ps(op, SCADYN_KCLASS_TGEMM, 8.0 * (double)ncol * ncol * (double)n);
-            launch_tgemm(mat, X, n, ncol, T, wk, work, s);
+            ProfScope ps(op, SCADYN_KCLASS_TGEMM, 8.0 * (double)ncol * ns * (double)n);
+            launch_tgemm(mat, X, n, ncol, ns, T, wk, work, s);
         }
-        std::vector<cd> hT((size_t)ncol * ncol);
+        std::vector<cd> hT((size_t)ncol * ns);
         HIP_CHECK(hipMemcpyAsync(hT.data(), T, sizeof(cd) * hT.size(), hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
-        // sc = i k^3; Taa = T(1:nm,1:nm)*sc etc. (src/T_matrix.f90:270-275)
+        // sc = i k^3 (src/T_matrix.f90:270-275): (x + i y)(i k3) = -y k3 + i x k3
         const double k3 = std::pow(op->k, 3.0);
+        for (auto &v : hT) v = cmk(-v.y * k3, v.x * k3);
+        op->last_outer_rounds = rounds;
+        return hT;
+    }
+
+    // the four nm x nm column-major blocks (all columns solved)
+    void result(double *Taa, double *Tab, double *Tba, double *Tbb, int32_t *it_out) {
+        const std::vector<cd> hT = scaled_T();
         double *outs[4] = {Taa, Tab, Tba, Tbb};
         for (int q = 0; q < 4; ++q) {
             const int r0 = (q >= 2) ? nm : 0, c0 = (q % 2) ? nm : 0;
             for (int j = 0; j < nm; ++j)
                 for (int i = 0; i < nm; ++i) {
-                    cd v = hT[(size_t)(r0 + i) * ncol + (c0 + j)];
-                    // (v.x + i v.y) * (0 + i k3) = (-v.y k3) + i (v.x k3)
-                    outs[q][2 * ((size_t)j * nm + i)] = -v.y * k3;
-                    outs[q][2 * ((size_t)j * nm + i) + 1] = v.x * k3;
+                    const cd v = hT[(size_t)(r0 + i) * ncol + (c0 + j)];
+                    outs[q][2 * ((size_t)j * nm + i)] = v.x;
+                    outs[q][2 * ((size_t)j * nm + i) + 1] = v.y;
                 }
         }
         if (it_out) std::copy(iters.begin(), iters.end(), it_out);
-        op->last_outer_rounds = rounds;
     }
+
+    // the solved columns: Tcols (ncol x ns) column-major; a column whose system
+    // has not finished yet reads as zero with iteration count -1
+    void result_cols(double *Tcols, int32_t *it_out) {
+        const std::vector<cd> hT = scaled_T();
+        for (int j = 0; j < ns; ++j) {
+            const bool fin = cs->col_fin[j] != 0;
+            for (int i = 0; i < ncol; ++i) {
+                Tcols[2 * ((size_t)j * ncol + i)] = fin ? hT[(size_t)i * ns + j].x : 0.0;
+                Tcols[2 * ((size_t)j * ncol + i) + 1] = fin ? hT[(size_t)i * ns + j].y : 0.0;
+            }
+            if (it_out) it_out[j] = fin ? iters[j] : -1;
+        }
+    }
+
+    int columns_done() const { return cs ? cs->ndone : 0; }
 };
 
 static void tmatrix(scadyn_operator *op, int nmax, double *Taa, double *Tab, double *Tba, double *Tbb,
@@ -1216,11 +1275,20 @@ struct Request {
     double k = 0;
     int nmax = 0;
     size_t fp = 0;
+    // state is written only under the server mutex: QUEUED -> RUNNING at
+    // admission, DONE / FAILED when the worker retires the request (the same
+    // critical section that removes it from the active list and returns its
+    // footprint).  A waiter that sees DONE / FAILED may free the request, so
+    // the worker never touches it after publishing.
     int state = REQ_QUEUED;
+    // the worker's own record of the outcome (REQ_DONE / REQ_FAILED, 0 while
+    // solving); read and written by the worker thread only
+    int outcome = 0;
     bool set_up = false;
+    int stub_steps = 0;  // host-only stand-in solve: steps left
     int err = 0;
     std::string msg;
-    double t[4] = {0, 0, 0, 0};  // submit, admit, solve start, done (s since server start)
+    double t[4] = {0, 0, 0, 0};  // submit, admit, solve start, done (s since server start); under the mutex
     std::unique_ptr<scadyn_operator> op;
     std::unique_ptr<TmatJob> job;
     std::vector<double> T;  // 4 blocks, column-major complex
@@ -1231,6 +1299,7 @@ struct Request {
 
 struct scadyn_server {
     int device = 0, max_active = 8;
+    bool host_only = false;  // admission and retirement with a host stand-in solve (no device)
     size_t budget = 0, in_use = 0;
     std::mutex mu;
     std::condition_variable cv, done_cv;
@@ -1249,20 +1318,25 @@ struct scadyn_server {
 
 namespace scadyn {
 
-static void fail_request(scadyn_server *sv, Request *r, int code, const std::string &msg) {
+// worker thread: record a failure; it is published at retirement
+static void fail_request(scadyn_server *, Request *r, int code, const std::string &msg) {
     r->err = code;
     r->msg = msg;
-    r->state = REQ_FAILED;
-    r->t[3] = sv->now();
+    r->outcome = REQ_FAILED;
     r->job.reset();
     r->op.reset();
+}
+
+static void stamp(scadyn_server *sv, Request *r, int i) {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    r->t[i] = sv->now();
 }
 
 }  // namespace scadyn
 
 void scadyn_server::run() {
     using namespace scadyn;
-    (void)hipSetDevice(device);
+    if (!host_only) (void)hipSetDevice(device);
     for (;;) {
         {
             std::unique_lock<std::mutex> lk(mu);
@@ -1282,17 +1356,24 @@ void scadyn_server::run() {
                 continue;
             }
         }
+        // The active list changes only in this thread (admission above and
+        // retirement below), so it is read here without the lock; a request
+        // in it is not freed before this thread retires it.
         // set up at most one admitted request per pass (host geometry + device setup)
         for (Request *r : active) {
-            if (r->set_up || r->state != REQ_RUNNING) continue;
+            if (r->set_up || r->outcome) continue;
             try {
-                scadyn_mesh m{r->n_node, r->n_tet, r->coord.data(), r->etopol.data(), r->param.data()};
-                r->op.reset(new scadyn_operator());
-                create_operator(r->op.get(), m, r->opts, r->k);
-                r->job.reset(new TmatJob());
-                r->job->start(r->op.get(), r->nmax);
+                if (host_only) {
+                    r->stub_steps = 3;
+                } else {
+                    scadyn_mesh m{r->n_node, r->n_tet, r->coord.data(), r->etopol.data(), r->param.data()};
+                    r->op.reset(new scadyn_operator());
+                    create_operator(r->op.get(), m, r->opts, r->k);
+                    r->job.reset(new TmatJob());
+                    r->job->start(r->op.get(), r->nmax);
+                }
                 r->set_up = true;
-                r->t[2] = now();
+                stamp(this, r, 2);
             } catch (const Error &e) {
                 fail_request(this, r, e.code, e.msg);
             }
@@ -1300,7 +1381,7 @@ void scadyn_server::run() {
         }
         // one step of every solving request: launch all, then complete all
         for (Request *r : active)
-            if (r->set_up && r->state == REQ_RUNNING) {
+            if (r->set_up && !r->outcome && !host_only) {
                 try {
                     r->job->launch();
                 } catch (const Error &e) {
@@ -1308,11 +1389,21 @@ void scadyn_server::run() {
                 }
             }
         for (Request *r : active)
-            if (r->set_up && r->state == REQ_RUNNING) {
+            if (r->set_up && !r->outcome) {
+                const int nm = (r->nmax + 1) * (r->nmax + 1) - 1;
+                if (host_only) {
+                    // stand-in solve for the loop's own tests: no T-matrix is
+                    // computed; the iteration counts carry the request id
+                    std::this_thread::sleep_for(std::chrono::milliseconds(1));
+                    if (--r->stub_steps > 0) continue;
+                    r->T.assign((size_t)8 * nm * nm, 0.0);
+                    r->iters.assign(2 * nm, (int32_t)r->id);
+                    r->outcome = REQ_DONE;
+                    continue;
+                }
                 try {
                     r->job->finish();
                     if (r->job->finished) {
-                        const int nm = (r->nmax + 1) * (r->nmax + 1) - 1;
                         r->T.assign((size_t)8 * nm * nm, 0.0);
                         r->iters.assign(2 * nm, 0);
                         const size_t blk = (size_t)2 * nm * nm;
@@ -1321,13 +1412,12 @@ void scadyn_server::run() {
                         const int nf = r->job->nfail;
                         r->job.reset();
                         r->op.reset();
-                        r->t[3] = now();
                         if (nf && r->opts.strict) {
                             r->err = SCADYN_ERR_NOCONV;
                             r->msg = std::to_string(nf) + " T-matrix columns did not reach tol";
-                            r->state = REQ_FAILED;
+                            r->outcome = REQ_FAILED;
                         } else {
-                            r->state = REQ_DONE;
+                            r->outcome = REQ_DONE;
                         }
                     }
                 } catch (const Error &e) {
@@ -1335,13 +1425,18 @@ void scadyn_server::run() {
                 }
             }
         {
+            // retire: publish the outcome, return the footprint and drop the
+            // request from the active list in one critical section
             std::lock_guard<std::mutex> lk(mu);
             std::vector<Request *> keep;
             for (Request *r : active) {
-                if (r->state == REQ_DONE || r->state == REQ_FAILED)
+                if (r->outcome) {
                     in_use -= r->fp;
-                else
+                    r->t[3] = now();
+                    r->state = r->outcome;  // r may be freed by a waiter from here on
+                } else {
                     keep.push_back(r);
+                }
             }
             if (keep.size() != active.size()) done_cv.notify_all();
             active.swap(keep);
@@ -1598,7 +1693,7 @@ int scadyn_op_project(scadyn_operator *op, int32_t nmax, const double *X, double
     cd *dT = tmp.alloc<cd>((size_t)ncol * ncol);
     {
         ProfScope ps(op, SCADYN_KCLASS_TGEMM, 8.0 * (double)ncol * ncol * (double)n);
-        launch_tgemm(mat, dx, (long long)n, ncol, dT, wk, work, op->s);
+        launch_tgemm(mat, dx, (long long)n, ncol, ncol, dT, wk, work, op->s);
     }
     std::vector<cd> hT((size_t)ncol * ncol);
     HIP_CHECK(hipMemcpyAsync(hT.data(), dT, sizeof(cd) * hT.size(), hipMemcpyDeviceToHost, op->s));
@@ -1618,6 +1713,75 @@ int scadyn_op_tmatrix(scadyn_operator *op, int32_t nmax, double *Taa, double *Ta
     ABI_CATCH
 }
 
+int scadyn_op_tmatrix_cols(scadyn_operator *op, int32_t nmax, int32_t ncols, const int32_t *cols, double *Tcols,
+                           int32_t *iters) {
+    ABI_TRY
+    if (!op || !cols || !Tcols || ncols < 1) fail(SCADYN_ERR_ARG, "bad argument");
+    HIP_CHECK(hipSetDevice(op->device));
+    std::vector<int> sel(cols, cols + ncols);
+    TmatJob job;
+    job.start(op, nmax, &sel);
+    while (!job.finished) {
+        job.launch();
+        job.finish();
+    }
+    job.result_cols(Tcols, iters);
+    if (job.nfail && op->opts.strict)
+        fail(SCADYN_ERR_NOCONV, std::to_string(job.nfail) + " T-matrix columns did not reach tol");
+    ABI_CATCH
+}
+
+}  // extern "C"
+
+struct scadyn_tjob {
+    scadyn_operator *op = nullptr;
+    scadyn::TmatJob job;
+};
+
+extern "C" {
+
+int scadyn_tjob_create(scadyn_operator *op, int32_t nmax, int32_t ncols, const int32_t *cols, scadyn_tjob **out) {
+    ABI_TRY
+    if (!op || !cols || !out || ncols < 1) fail(SCADYN_ERR_ARG, "bad argument");
+    *out = nullptr;
+    HIP_CHECK(hipSetDevice(op->device));
+    std::unique_ptr<scadyn_tjob> j(new scadyn_tjob());
+    j->op = op;
+    std::vector<int> sel(cols, cols + ncols);
+    j->job.start(op, nmax, &sel);
+    *out = j.release();
+    ABI_CATCH
+}
+
+int scadyn_tjob_advance(scadyn_tjob *j, int32_t want, int32_t *ndone) {
+    ABI_TRY
+    if (!j) fail(SCADYN_ERR_ARG, "null job");
+    HIP_CHECK(hipSetDevice(j->op->device));
+    const int target = j->job.columns_done() + std::max(want, 0);
+    while (!j->job.finished && j->job.columns_done() < target) {
+        j->job.launch();
+        j->job.finish();
+    }
+    HIP_CHECK(hipStreamSynchronize(j->op->s));
+    if (ndone) *ndone = j->job.columns_done();
+    ABI_CATCH
+}
+
+int scadyn_tjob_result(scadyn_tjob *j, double *Tcols, int32_t *iters) {
+    ABI_TRY
+    if (!j || !Tcols) fail(SCADYN_ERR_ARG, "bad argument");
+    HIP_CHECK(hipSetDevice(j->op->device));
+    j->job.result_cols(Tcols, iters);
+    ABI_CATCH
+}
+
+void scadyn_tjob_destroy(scadyn_tjob *j) {
+    if (!j) return;
+    (void)hipSetDevice(j->op->device);
+    (void)hipStreamSynchronize(j->op->s);
+    delete j;
+}
+
 int scadyn_tmat_solve(const scadyn_mesh *mesh, const scadyn_opts *opts, double k, int32_t nmax, double *Taa,
                       double *Tab, double *Tba, double *Tbb, int32_t *iters) {
     scadyn_operator *op = nullptr;
@@ -1632,7 +1796,7 @@ int scadyn_op_profile(scadyn_operator *op, int32_t enable) {
     ABI_TRY
     if (!op) fail(SCADYN_ERR_ARG, "null operator");
     op->prof.on = enable != 0;
-    for (int i = 0; i < SCADYN_KCLASS_COUNT; ++i) op->prof.ms[i] = op->prof.cnt[i] = op->prof.byt[i] = 0;
+    for (int i = 0; i < 2 * SCADYN_KCLASS_COUNT; ++i) op->prof.ms[i] = op->prof.cnt[i] = op->prof.byt[i] = 0;
     ABI_CATCH
 }
 
@@ -1641,9 +1805,26 @@ int scadyn_op_timings(scadyn_operator *op, double *t) {
     if (!op || !t) fail(SCADYN_ERR_ARG, "null argument");
     op->prof.collect();
     for (int i = 0; i < SCADYN_KCLASS_COUNT; ++i) {
-        t[3 * i] = op->prof.ms[i];
-        t[3 * i + 1] = op->prof.cnt[i];
-        t[3 * i + 2] = op->prof.byt[i];
+        const int j = i + SCADYN_KCLASS_COUNT;
+        t[3 * i] = op->prof.ms[i] + op->prof.ms[j];
+        t[3 * i + 1] = op->prof.cnt[i] + op->prof.cnt[j];
+        t[3 * i + 2] = op->prof.byt[i] + op->prof.byt[j];
+    }
+    ABI_CATCH
+}
+
+int scadyn_op_timings_prec(scadyn_operator *op, int32_t precision, double *t) {
+    ABI_TRY
+    if (!op || !t) fail(SCADYN_ERR_ARG, "null argument");
+    if (precision != SCADYN_PREC_FP64 && precision != SCADYN_PREC_MIXED) fail(SCADYN_ERR_ARG, "bad precision");
+    op->prof.collect();
+    const int po = precision == SCADYN_PREC_MIXED ? SCADYN_KCLASS_COUNT : 0;
+    for (int i = 0; i < SCADYN_KCLASS_COUNT; ++i) {
+        const bool mv = i < SCADYN_KCLASS_KRYLOV;  // matvec classes split by precision
+        const int j = i + SCADYN_KCLASS_COUNT;
+        t[3 * i] = mv ? op->prof.ms[i + po] : op->prof.ms[i] + op->prof.ms[j];
+        t[3 * i + 1] = mv ? op->prof.cnt[i + po] : op->prof.cnt[i] + op->prof.cnt[j];
+        t[3 * i + 2] = mv ? op->prof.byt[i + po] : op->prof.byt[i] + op->prof.byt[j];
     }
     ABI_CATCH
 }
@@ -1672,7 +1853,9 @@ int scadyn_server_create(const scadyn_server_opts *so, scadyn_server **out) {
     std::unique_ptr<scadyn_server> sv(new scadyn_server());
     sv->device = so ? so->device : 0;
     sv->max_active = (so && so->max_active > 0) ? so->max_active : 8;
+    sv->host_only = so && so->host_only != 0;
     size_t budget = so ? (size_t)std::max<int64_t>(so->hbm_budget, 0) : 0;
+    if (budget == 0 && sv->host_only) fail(SCADYN_ERR_ARG, "a host-only server needs an explicit hbm_budget");
     if (budget == 0) {
         HIP_CHECK(hipSetDevice(sv->device));
         size_t fr = 0, tot = 0;

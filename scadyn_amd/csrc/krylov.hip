@@ -725,7 +725,7 @@ __global__ void k_retire(KrylovDims d, const int *__restrict__ col, SlotState *_
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(256) void k_tgemm(const cd *__restrict__ A, const cd *__restrict__ X, long long n,
-                                               int ncol, long long chunk, cd *__restrict__ part) {
+                                               int na, int nx, long long chunk, cd *__restrict__ part) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int kk = lane >> 4, ii = lane & 15;
     const int i0 = blockIdx.x * 64 + (wv >> 1) * 32, j0 = blockIdx.y * 64 + (wv & 1) * 32;
@@ -740,10 +740,10 @@ __global__ __launch_bounds__(256) void k_tgemm(const cd *__restrict__ A, const c
     for (long long r = r0; r < r1; r += 4) {
         const long long row = r + kk;
         const bool rok = row < r1;
-        const cd *ar = A + (size_t)row * ncol, *xr = X + (size_t)row * ncol;
+        const cd *ar = A + (size_t)row * na, *xr = X + (size_t)row * nx;
         const cd z = cmk(0.0, 0.0);
-        const cd a0 = (rok && ca0 < ncol) ? ar[ca0] : z, a1 = (rok && ca1 < ncol) ? ar[ca1] : z;
-        const cd b0 = (rok && cb0 < ncol) ? xr[cb0] : z, b1 = (rok && cb1 < ncol) ? xr[cb1] : z;
+        const cd a0 = (rok && ca0 < na) ? ar[ca0] : z, a1 = (rok && ca1 < na) ? ar[ca1] : z;
+        const cd b0 = (rok && cb0 < nx) ? xr[cb0] : z, b1 = (rok && cb1 < nx) ? xr[cb1] : z;
         const double am[2][2] = {{a0.x, a0.y}, {a1.x, a1.y}};
         const double bm[2][2] = {{b0.x, b0.y}, {b1.x, b1.y}};
 #pragma unroll
@@ -756,7 +756,7 @@ __global__ __launch_bounds__(256) void k_tgemm(const cd *__restrict__ A, const c
                 accI[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-am[a][1], bm[b][0], accI[a][b], 0, 0, 0);
             }
     }
-    cd *out = part + (size_t)blockIdx.z * ncol * ncol;
+    cd *out = part + (size_t)blockIdx.z * na * nx;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -764,12 +764,11 @@ __global__ __launch_bounds__(256) void k_tgemm(const cd *__restrict__ A, const c
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int i = i0 + 16 * a + kk + 4 * e, j = j0 + 16 * b + ii;
-                if (i < ncol && j < ncol) out[(size_t)i * ncol + j] = cmk(accR[a][b][e], accI[a][b][e]);
+                if (i < na && j < nx) out[(size_t)i * nx + j] = cmk(accR[a][b][e], accI[a][b][e]);
             }
 }
 
-__global__ void k_tgemm_reduce(const cd *__restrict__ part, int nz, int ncol, cd *__restrict__ T) {
-    const size_t tot = (size_t)ncol * ncol;
+__global__ void k_tgemm_reduce(const cd *__restrict__ part, int nz, size_t tot, cd *__restrict__ T) {
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
         cd a = cmk(0, 0);
         for (int z = 0; z < nz; ++z) a = cadd(a, part[(size_t)z * tot + e]);
@@ -914,20 +913,21 @@ void launch_retire(const KrylovDims &d, const int *col, SlotState *st, hipStream
     hipLaunchKernelGGL(k_retire, dim3((d.B + 63) / 64), dim3(64), 0, s, d, col, st);
     HIP_CHECK(hipGetLastError());
 }
-void launch_tgemm(const cd *mat, const cd *X, long long n, int ncol, cd *T, cd *work, size_t work_elems,
+void launch_tgemm(const cd *mat, const cd *X, long long n, int na, int nx, cd *T, cd *work, size_t work_elems,
                   hipStream_t s) {
     // 64 x 64 tiles; row chunks (multiples of 4 rows) so that about 2048
     // workgroups run, limited by the partial-sum workspace
-    const int tiles = (ncol + 63) / 64;
-    long long nz = std::max<long long>(1, 2048 / (tiles * tiles));
-    nz = std::min<long long>(nz, std::max<long long>(1, (long long)(work_elems / ((size_t)ncol * ncol))));
+    const int ta = (na + 63) / 64, tx = (nx + 63) / 64;
+    const size_t tot = (size_t)na * nx;
+    long long nz = std::max<long long>(1, 2048 / (ta * tx));
+    nz = std::min<long long>(nz, std::max<long long>(1, (long long)(work_elems / tot)));
     nz = std::min<long long>(nz, std::max<long long>(1, n / 256));
     long long chunk = (n + nz - 1) / nz;
     chunk = (chunk + 3) / 4 * 4;
     nz = (n + chunk - 1) / chunk;
-    hipLaunchKernelGGL(k_tgemm, dim3(tiles, tiles, (unsigned)nz), dim3(256), 0, s, mat, X, n, ncol, chunk, work);
+    hipLaunchKernelGGL(k_tgemm, dim3(ta, tx, (unsigned)nz), dim3(256), 0, s, mat, X, n, na, nx, chunk, work);
     HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_tgemm_reduce, dim3(256), dim3(256), 0, s, work, (int)nz, ncol, T);
+    hipLaunchKernelGGL(k_tgemm_reduce, dim3(256), dim3(256), 0, s, work, (int)nz, tot, T);
     HIP_CHECK(hipGetLastError());
 }
 

@@ -113,8 +113,9 @@ void launch_unpack_slots(const KrylovDims &d, const cd *in, const int *slot, int
 // mark slots with col[b] >= 0 idle
 void launch_retire(const KrylovDims &d, const int *col, SlotState *st, hipStream_t s);
 
-// T = mat^H X: mat, X are n x ncol row-major; T is ncol x ncol row-major (i, j)
-void launch_tgemm(const cd *mat, const cd *X, long long n, int ncol, cd *T, cd *work, size_t work_elems,
+// T = mat^H X: mat is n x na, X n x nx (row-major); T is na x nx row-major (i, j).
+// work holds the row-chunk partials (at least na*nx elements).
+void launch_tgemm(const cd *mat, const cd *X, long long n, int na, int nx, cd *T, cd *work, size_t work_elems,
                   hipStream_t s);
 
 }  // namespace scadyn

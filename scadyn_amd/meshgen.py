@@ -96,6 +96,24 @@ def sphere(n_cells=24, radius=1.0, jitter=0.1, seed=0):
     return cube_split(inside, -radius, radius, n_cells, jitter, seed)
 
 
+def ball(n_cells=8, radius=1.0, seed=0, jitter=0.0):
+    """Sphere with a faceted (not staircase) surface: the whole cube-split
+    lattice of [-r, r]^3, every vertex moved radially by |p|_inf / |p|_2 so that
+    the cube's concentric shells become spheres.  The map is a positive radial
+    scaling (Jacobian determinant s^3 > 0 with s = |p|_inf / |p|_2), so the
+    tets keep their orientation; boundary vertices lie on the sphere and the
+    surface error is that of flat facets, O(h^2)."""
+    c, e = cube_split(lambda p: np.ones(len(p), bool), -radius, radius, n_cells, jitter, seed)
+    r2 = np.sqrt(np.sum(c * c, axis=1))
+    s = np.where(r2 > 0, np.max(np.abs(c), axis=1) / np.maximum(r2, 1e-300), 1.0)
+    c = c * s[:, None]
+    p = c[e]
+    det = np.einsum('ni,ni->n', p[:, 3] - p[:, 0], np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]))
+    if (det <= 0).any():
+        raise ValueError('ball: a mapped tet lost its orientation')
+    return c, e
+
+
 def _real_sh_series(lmax, coeffs, theta, phi):
     """s(theta,phi) = sum_{l,m} P_l^m(cos t) (a_lm cos m phi + b_lm sin m phi)."""
     from scipy.special import lpmv

@@ -20,8 +20,10 @@ KCLASS = ['scatter', 'fftx', 'ffty', 'fftz_conv', 'iffty', 'ifftx', 'gather_near
 # every symbol include/scadyn_amd.h declares
 EXPORTED = ['scadyn_opts_default', 'scadyn_last_error', 'scadyn_tmat_solve', 'scadyn_op_create',
             'scadyn_op_destroy', 'scadyn_op_info', 'scadyn_op_export', 'scadyn_op_matvec', 'scadyn_op_matvec_prec',
-            'scadyn_op_solve', 'scadyn_op_vswf', 'scadyn_op_project', 'scadyn_op_tmatrix', 'scadyn_op_profile',
-            'scadyn_op_timings', 'scadyn_footprint', 'scadyn_server_create', 'scadyn_server_submit',
+            'scadyn_op_solve', 'scadyn_op_vswf', 'scadyn_op_project', 'scadyn_op_tmatrix', 'scadyn_op_tmatrix_cols',
+            'scadyn_tjob_create', 'scadyn_tjob_advance', 'scadyn_tjob_result', 'scadyn_tjob_destroy',
+            'scadyn_op_profile',
+            'scadyn_op_timings', 'scadyn_op_timings_prec', 'scadyn_footprint', 'scadyn_server_create', 'scadyn_server_submit',
             'scadyn_server_status', 'scadyn_server_wait', 'scadyn_server_destroy']
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 
@@ -41,7 +43,7 @@ class CMesh(ctypes.Structure):
 
 class CServerOpts(ctypes.Structure):
     _fields_ = [('device', ctypes.c_int32), ('max_active', ctypes.c_int32), ('hbm_budget', ctypes.c_int64),
-                ('reserved', ctypes.c_int32 * 8)]
+                ('host_only', ctypes.c_int32), ('reserved', ctypes.c_int32 * 7)]
 
 
 class COpts(ctypes.Structure):
@@ -84,8 +86,15 @@ def lib():
     L.scadyn_op_vswf.argtypes = [P, ctypes.c_int32, dp]
     L.scadyn_op_project.argtypes = [P, ctypes.c_int32, dp, dp]
     L.scadyn_op_tmatrix.argtypes = [P, ctypes.c_int32, dp, dp, dp, dp, ip]
+    L.scadyn_op_tmatrix_cols.argtypes = [P, ctypes.c_int32, ctypes.c_int32, ip, dp, ip]
+    L.scadyn_tjob_create.argtypes = [P, ctypes.c_int32, ctypes.c_int32, ip, ctypes.POINTER(P)]
+    L.scadyn_tjob_advance.argtypes = [P, ctypes.c_int32, ip]
+    L.scadyn_tjob_result.argtypes = [P, dp, ip]
+    L.scadyn_tjob_destroy.argtypes = [P]
+    L.scadyn_tjob_destroy.restype = None
     L.scadyn_op_profile.argtypes = [P, ctypes.c_int32]
     L.scadyn_op_timings.argtypes = [P, dp]
+    L.scadyn_op_timings_prec.argtypes = [P, ctypes.c_int32, dp]
     L.scadyn_footprint.argtypes = [ctypes.POINTER(CMesh), ctypes.POINTER(COpts), ctypes.c_int32,
                                    ctypes.POINTER(ctypes.c_int64)]
     L.scadyn_server_create.argtypes = [ctypes.POINTER(CServerOpts), ctypes.POINTER(P)]
@@ -96,7 +105,8 @@ def lib():
     L.scadyn_server_destroy.argtypes = [P]
     L.scadyn_server_destroy.restype = None
     for name in EXPORTED:
-        if name not in ('scadyn_opts_default', 'scadyn_last_error', 'scadyn_op_destroy', 'scadyn_server_destroy'):
+        if name not in ('scadyn_opts_default', 'scadyn_last_error', 'scadyn_op_destroy', 'scadyn_server_destroy',
+                        'scadyn_tjob_destroy'):
             getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -236,14 +246,64 @@ class Operator:
         check(lib().scadyn_op_tmatrix(self.h, nmax, *[_dp(t.view(np.float64)) for t in T], _ip(it)))
         return tuple(t.T.copy() for t in T), it
 
+    def tmatrix_cols(self, nmax, cols):
+        """T(:, cols) (2nm x len(cols)), solving only those columns' systems."""
+        nm = (nmax + 1) ** 2 - 1
+        sel = np.ascontiguousarray(np.asarray(cols), dtype=np.int32)
+        T = np.zeros((sel.size, 2 * nm), np.complex128)  # filled column-major
+        it = np.zeros(sel.size, np.int32)
+        check(lib().scadyn_op_tmatrix_cols(self.h, nmax, sel.size, _ip(sel), _dp(T.view(np.float64)), _ip(it)))
+        return T.T.copy(), it
+
     def profile(self, on=True):
         check(lib().scadyn_op_profile(self.h, 1 if on else 0))
 
-    def timings(self):
+    def timings(self, precision=None):
+        """Per-class totals; precision 'fp64' / 'mixed' (or PREC_*) restricts the
+        matvec classes to the launches of that matvec precision."""
         t = np.zeros((len(KCLASS), 3))
-        check(lib().scadyn_op_timings(self.h, _dp(t)))
+        if precision is None:
+            check(lib().scadyn_op_timings(self.h, _dp(t)))
+        else:
+            p = {'fp64': PREC_FP64, 'mixed': PREC_MIXED}.get(precision, precision)
+            check(lib().scadyn_op_timings_prec(self.h, p, _dp(t)))
         return {name: {'ms': t[i, 0], 'launches': int(t[i, 1]), 'bytes': t[i, 2]}
                 for i, name in enumerate(KCLASS)}
+
+
+class TJob:
+    """Resumable T-matrix solve over a column list (scadyn_tjob_*)."""
+
+    def __init__(self, op, nmax, cols):
+        self.op = op  # keeps the operator alive
+        self.nmax = nmax
+        self.cols = np.ascontiguousarray(np.asarray(cols), dtype=np.int32)
+        h = ctypes.c_void_p()
+        check(lib().scadyn_tjob_create(op.h, nmax, self.cols.size, _ip(self.cols), ctypes.byref(h)))
+        self.h = h
+
+    def advance(self, want):
+        nd = np.zeros(1, np.int32)
+        check(lib().scadyn_tjob_advance(self.h, int(want), _ip(nd)))
+        return int(nd[0])
+
+    def result(self):
+        nm = (self.nmax + 1) ** 2 - 1
+        T = np.zeros((self.cols.size, 2 * nm), np.complex128)
+        it = np.zeros(self.cols.size, np.int32)
+        check(lib().scadyn_tjob_result(self.h, _dp(T.view(np.float64)), _ip(it)))
+        return T.T.copy(), it
+
+    def close(self):
+        if getattr(self, 'h', None):
+            lib().scadyn_tjob_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def tmat_solve(coord, etopol, param, k, nmax, **opts):
@@ -270,8 +330,8 @@ def footprint(coord, etopol, param, nmax, **opts):
 class Server:
     """The library's request loop: submit T-matrix requests, wait for results."""
 
-    def __init__(self, device=0, max_active=8, hbm_budget=0):
-        so = CServerOpts(device, max_active, hbm_budget)
+    def __init__(self, device=0, max_active=8, hbm_budget=0, host_only=False):
+        so = CServerOpts(device, max_active, hbm_budget, 1 if host_only else 0)
         h = ctypes.c_void_p()
         check(lib().scadyn_server_create(ctypes.byref(so), ctypes.byref(h)))
         self.h = h

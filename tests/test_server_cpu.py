@@ -1,5 +1,6 @@
 """Request-loop admission on the host: footprint model and over-budget rejection
 (no device needed: requests are validated and sized before they are queued)."""
+import numpy as np
 import pytest
 
 from helpers import sphere_case
@@ -32,3 +33,40 @@ def test_over_budget_request_is_rejected_not_queued():
         sv.submit(c, e, p, k, 2)
     assert ex.value.code == native.ERR_MEMORY
     sv.close()
+
+
+def test_host_only_loop_admits_and_retires_concurrently():
+    """Submit and wait from several threads against the loop's host-only
+    stand-in solve: every request is admitted within the budget, retired once,
+    and its waiter gets its own record (the stand-in's iteration counts carry
+    the request id).  Waiters free their requests while the worker keeps
+    stepping others, the pattern that a publish-before-retire race breaks."""
+    import threading
+    from scadyn_amd import native
+    c, e, p, k = sphere_case()
+    fp = native.footprint(c, e, p, 1)
+    # room for two requests at a time
+    sv = native.Server(max_active=8, hbm_budget=int(2.5 * fp), host_only=True)
+    errors = []
+    lock = threading.Lock()
+
+    def client(nreq):
+        try:
+            for _ in range(nreq):
+                with lock:
+                    rid = sv.submit(c, e, p, k, 1)
+                T, it, times = sv.wait(rid)
+                assert (it == rid).all(), (rid, it)
+                assert times[0] <= times[1] <= times[2] <= times[3]
+                assert all(np.linalg.norm(t) == 0 for t in T)
+        except Exception as ex:  # pragma: no cover - reported below
+            errors.append(repr(ex))
+
+    threads = [threading.Thread(target=client, args=(6,)) for _ in range(6)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    sv.close()
+    assert not errors, errors
+    assert not any(t.is_alive() for t in threads)
