@@ -18,11 +18,14 @@ The relative Frobenius difference of the two legs' T columns is reported.
 
 A step is K converged columns (default K = 4): the batch keeps every slot busy
 and refills a slot as soon as its column converges, so a step is the time in
-which K more columns finish.  --warmup steps run first (the first also fills
-the pipeline: no column converges before about 700 solver iterations), then
---steps steps are timed, with a device synchronisation on both sides.  `value`
+which K more columns finish.  The first fills of the batch's slots are spread
+over 800 solver steps (about one column's iterations), so that columns finish
+at an even rate rather than in batch-wide bursts.  --warmup steps run first
+(the first also fills the pipeline: no column converges before about 700
+solver iterations), then --steps steps are timed, with a device
+synchronisation on both sides.  `value`
 is the mixed leg's converged columns per second over the timed steps.  The
-FP64 leg runs --fp64-warmup and --fp64-steps steps of the same kind.
+FP64 leg runs --fp64-warmup (5) and --fp64-steps (10) steps of the same kind.
 
 Per-kernel times come from the library's HIP-event profiler on the operator's
 stream over the timed steps; `roofline` names the matvec kernel with the
@@ -92,10 +95,10 @@ def kernel_table(tim):
     return out
 
 
-def run_leg(name, op, nmax, order, K, warmup, steps, prec_key):
+def run_leg(name, op, nmax, order, K, warmup, steps, prec_key, spread):
     """warmup + timed steps of K converged columns on one operator."""
     from scadyn_amd import native
-    job = native.TJob(op, nmax, order)
+    job = native.TJob(op, nmax, order, spread=spread)
     t0 = time.time()
     done = 0
     for w in range(warmup):
@@ -163,7 +166,7 @@ def headline(args):
             warm, steps = args.fp64_warmup, args.fp64_steps
         setup = time.time() - t
         say(f'{name} operator set up in {setup:.1f} s')
-        r, T, it, tim = run_leg(name, op, nmax, order, args.cols_per_step, warm, steps, name)
+        r, T, it, tim = run_leg(name, op, nmax, order, args.cols_per_step, warm, steps, name, args.spread)
         r['setup_s'] = setup
         info = op.info()
         r['roofline'] = roofline(tim, args.workload, name, info['batch'])
@@ -309,8 +312,11 @@ def main():
     ap.add_argument('--warmup', type=int, default=5)
     ap.add_argument('--workload', default='layered824k')
     ap.add_argument('--cols-per-step', type=int, default=4, help='converged columns per step')
-    ap.add_argument('--fp64-steps', type=int, default=3)
-    ap.add_argument('--fp64-warmup', type=int, default=1)
+    ap.add_argument('--fp64-steps', type=int, default=10)
+    ap.add_argument('--fp64-warmup', type=int, default=5)
+    ap.add_argument('--spread', type=int, default=800,
+                    help='solver steps over which the first fills of the batch are spread (about the '
+                         'iterations of one column): columns then finish at an even rate')
     ap.add_argument('--batch', type=int, default=0)
     ap.add_argument('--nmax', type=int, default=0, help='override the workload expansion order')
     ap.add_argument('--tol', type=float, default=1e-8,

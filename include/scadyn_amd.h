@@ -186,9 +186,13 @@ int scadyn_op_tmatrix_cols(scadyn_operator *op, int32_t nmax, int32_t ncols, con
  * advance runs solver steps until at least `want` more systems have finished
  * (or all have) and reports how many have finished in total; result writes
  * T(:, cols[j]) as scadyn_op_tmatrix_cols does for finished columns and zero
- * with iters[j] = -1 for the rest.  The operator must outlive the job. */
+ * with iters[j] = -1 for the rest.  spread > 0 spreads the first fills of the
+ * batch's B slots over that many solver steps (slot b starts at step
+ * b * spread / B): with spread about the iterations of one system, columns
+ * finish at an even rate from the first one on instead of in batch-wide
+ * bursts.  The operator must outlive the job. */
 typedef struct scadyn_tjob scadyn_tjob;
-int scadyn_tjob_create(scadyn_operator *op, int32_t nmax, int32_t ncols, const int32_t *cols,
+int scadyn_tjob_create(scadyn_operator *op, int32_t nmax, int32_t ncols, const int32_t *cols, int32_t spread,
                        scadyn_tjob **out);
 int scadyn_tjob_advance(scadyn_tjob *job, int32_t want, int32_t *ndone);
 int scadyn_tjob_result(scadyn_tjob *job, double *Tcols, int32_t *iters);
@@ -259,6 +263,11 @@ int scadyn_server_status(scadyn_server *sv, int64_t id, int32_t *state, double t
  * iteration counts and the times; releases the request. Returns its error. */
 int scadyn_server_wait(scadyn_server *sv, int64_t id, double *Taa, double *Tab, double *Tba, double *Tbb,
                        int32_t *iters, double times[4]);
+/* Cancels a request: a queued one leaves the queue at once, an admitted one is
+ * ended by the worker before its next solver step (its device memory and
+ * footprint are released); either then waits as FAILED ("request cancelled").
+ * A finished request is unaffected. */
+int scadyn_server_cancel(scadyn_server *sv, int64_t id);
 void scadyn_server_destroy(scadyn_server *sv);
 
 #ifdef __cplusplus

@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -838,6 +839,9 @@ struct ColumnSolver {
     std::vector<char> col_fin;  // 1 once column c (index into src/dst) has been retired
     int next = 0, nfail = 0, singular = 0, max_cycles = 0;
     long long steps = 0, max_steps = 0;
+    // first fills spread over `spread` solver steps: slot b takes its first
+    // column no earlier than step b * spread / B (0: all slots at once)
+    long long spread = 0;
     hipEvent_t ev = nullptr;
     bool pending = false;
 
@@ -886,7 +890,8 @@ struct ColumnSolver {
             HIP_CHECK(hipMemsetAsync(w->V, 0, sizeof(cd) * (size_t)(w->m + 2) * nB, op->s));
         }
         // every system finishes within restart cycles of at most m+1 matvecs; bound the loop
-        max_steps = ((long long)spec.restart * (w->m + 1) + 4) * (((long long)cols.size() + B - 1) / B + 2) + 16;
+        max_steps = ((long long)spec.restart * (w->m + 1) + 4) * (((long long)cols.size() + B - 1) / B + 2) + 16 +
+                    spread;
         HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     }
 
@@ -904,7 +909,7 @@ struct ColumnSolver {
         bool any_load = false;
         for (int b = 0; b < B; ++b) {
             h_load.p[b] = -1;
-            if (slot_col[b] < 0 && next < (int)cols.size()) {
+            if (slot_col[b] < 0 && next < (int)cols.size() && steps - 1 >= (long long)b * spread / B) {
                 h_load.p[b] = cols[next++];
                 slot_col[b] = h_load.p[b];
                 any_load = true;
@@ -1116,7 +1121,7 @@ struct TmatJob {
     bool finished = false;
     std::unique_ptr<ColumnSolver> cs;
 
-    void start(scadyn_operator *o, int nmax_, const std::vector<int> *subset = nullptr) {
+    void start(scadyn_operator *o, int nmax_, const std::vector<int> *subset = nullptr, int spread = 0) {
         op = o;
         nmax = nmax_;
         if (nmax < 1) fail(SCADYN_ERR_ARG, "nmax must be >= 1");
@@ -1146,6 +1151,7 @@ struct TmatJob {
         std::vector<int> all(ns);
         for (int c = 0; c < ns; ++c) all[c] = c;
         cs.reset(new ColumnSolver());
+        cs->spread = std::max(spread, 0);
         cs->init(op, src, ns, all, X, iters.data(), solve_spec(op));
     }
 
@@ -1253,12 +1259,24 @@ static Footprint footprint_of(const HostGeometry &g, long long P, const scadyn_o
            8 * 2 * (size_t)P + cdb * H[0] * H[1] * H[2] + cdb * (L[0] + L[1] + L[2]);
     if (o.precision == SCADYN_PREC_MIXED) f.op += 864 * N + 8 * H[0] * H[1] * H[2];
     f.setup = 2 * cdb * L[0] * L[1] * L[2] + 9 * (size_t)P + cdb * 27 * 27 * 27 + cdb * 98 * 27;
+    // solve workspace (workspace(), restart_space()): per slot the Krylov
+    // vectors (a complex64 basis for the mixed-precision blocked MGS) and two
+    // complex128-sized grids; the reduction partials of up to 2048 row-sweep
+    // workgroups; the mixed solve's narrow FP64 restart batch
     const int m = o.maxit;
+    const bool mixed = o.precision == SCADYN_PREC_MIXED;
+    const bool vf = mixed && o.ortho == 0 && o.mgs_block != 1;
     const size_t grid = L[0] * L[1] * (size_t)g.Nz * 4;
-    const size_t ws = (size_t)B * cdb * ((size_t)(m + 4) * n + 2 * grid) + (size_t)B * 1024 * (m + 8);
+    const size_t vec = vf ? (size_t)(m + 1) * n * sizeof(cf) + n * (3 * cdb + sizeof(cf)) : (size_t)(m + 4) * n * cdb;
+    size_t ws = (size_t)B * (vec + 2 * grid * cdb) + (size_t)B * 1024 * (m + 8) +
+                (size_t)2048 * (size_t)std::max(m + 1, 36) * B * cdb;
+    if (mixed) {
+        const size_t rc = (size_t)std::min(B, 8);
+        ws += rc * (2 * grid * cdb + 2 * n * cdb);
+    }
+    // right-hand sides and solutions (mat, X of the T-matrix job) and the GEMM partials
     const size_t ncol = 2 * ((size_t)(nmax + 1) * (nmax + 1) - 1);
-    size_t rhs = 2 * cdb * n * ncol + cdb * ncol * ncol * 65;
-    if (o.precision == SCADYN_PREC_MIXED) rhs += 2 * cdb * n * ncol;
+    const size_t rhs = 2 * cdb * n * ncol + cdb * ncol * ncol * 65;
     f.solve = ws + rhs;
     return f;
 }
@@ -1285,6 +1303,7 @@ struct Request {
     // solving); read and written by the worker thread only
     int outcome = 0;
     bool set_up = false;
+    std::atomic<bool> cancel{false};  // set by scadyn_server_cancel; the worker ends the request
     int stub_steps = 0;  // host-only stand-in solve: steps left
     int err = 0;
     std::string msg;
@@ -1359,12 +1378,14 @@ void scadyn_server::run() {
         // The active list changes only in this thread (admission above and
         // retirement below), so it is read here without the lock; a request
         // in it is not freed before this thread retires it.
+        for (Request *r : active)
+            if (!r->outcome && r->cancel.load()) fail_request(this, r, SCADYN_ERR_ARG, "request cancelled");
         // set up at most one admitted request per pass (host geometry + device setup)
         for (Request *r : active) {
             if (r->set_up || r->outcome) continue;
             try {
                 if (host_only) {
-                    r->stub_steps = 3;
+                    r->stub_steps = 20;
                 } else {
                     scadyn_mesh m{r->n_node, r->n_tet, r->coord.data(), r->etopol.data(), r->param.data()};
                     r->op.reset(new scadyn_operator());
@@ -1740,7 +1761,8 @@ struct scadyn_tjob {
 
 extern "C" {
 
-int scadyn_tjob_create(scadyn_operator *op, int32_t nmax, int32_t ncols, const int32_t *cols, scadyn_tjob **out) {
+int scadyn_tjob_create(scadyn_operator *op, int32_t nmax, int32_t ncols, const int32_t *cols, int32_t spread,
+                       scadyn_tjob **out) {
     ABI_TRY
     if (!op || !cols || !out || ncols < 1) fail(SCADYN_ERR_ARG, "bad argument");
     *out = nullptr;
@@ -1748,7 +1770,7 @@ int scadyn_tjob_create(scadyn_operator *op, int32_t nmax, int32_t ncols, const i
     std::unique_ptr<scadyn_tjob> j(new scadyn_tjob());
     j->op = op;
     std::vector<int> sel(cols, cols + ncols);
-    j->job.start(op, nmax, &sel);
+    j->job.start(op, nmax, &sel, spread);
     *out = j.release();
     ABI_CATCH
 }
@@ -1948,6 +1970,31 @@ int scadyn_server_wait(scadyn_server *sv, int64_t id, double *Taa, double *Tab, 
     for (int q = 0; q < 4; ++q)
         if (outs[q]) std::copy(r->T.begin() + q * blk, r->T.begin() + (q + 1) * blk, outs[q]);
     if (iters) std::copy(r->iters.begin(), r->iters.end(), iters);
+    ABI_CATCH
+}
+
+int scadyn_server_cancel(scadyn_server *sv, int64_t id) {
+    ABI_TRY
+    if (!sv) fail(SCADYN_ERR_ARG, "null server");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    auto it = sv->all.find(id);
+    if (it == sv->all.end()) fail(SCADYN_ERR_ARG, "unknown request id");
+    scadyn::Request *r = it->second.get();
+    if (r->state == scadyn::REQ_QUEUED) {
+        // not admitted yet: leaves the queue at once
+        for (auto q = sv->queue.begin(); q != sv->queue.end(); ++q)
+            if (*q == r) {
+                sv->queue.erase(q);
+                break;
+            }
+        r->err = SCADYN_ERR_ARG;
+        r->msg = "request cancelled";
+        r->t[3] = sv->now();
+        r->state = scadyn::REQ_FAILED;
+        sv->done_cv.notify_all();
+    } else if (r->state == scadyn::REQ_RUNNING) {
+        r->cancel.store(true);  // the worker ends it before its next step
+    }
     ABI_CATCH
 }
 

@@ -24,7 +24,7 @@ EXPORTED = ['scadyn_opts_default', 'scadyn_last_error', 'scadyn_tmat_solve', 'sc
             'scadyn_tjob_create', 'scadyn_tjob_advance', 'scadyn_tjob_result', 'scadyn_tjob_destroy',
             'scadyn_op_profile',
             'scadyn_op_timings', 'scadyn_op_timings_prec', 'scadyn_footprint', 'scadyn_server_create', 'scadyn_server_submit',
-            'scadyn_server_status', 'scadyn_server_wait', 'scadyn_server_destroy']
+            'scadyn_server_status', 'scadyn_server_wait', 'scadyn_server_cancel', 'scadyn_server_destroy']
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 
 
@@ -87,7 +87,7 @@ def lib():
     L.scadyn_op_project.argtypes = [P, ctypes.c_int32, dp, dp]
     L.scadyn_op_tmatrix.argtypes = [P, ctypes.c_int32, dp, dp, dp, dp, ip]
     L.scadyn_op_tmatrix_cols.argtypes = [P, ctypes.c_int32, ctypes.c_int32, ip, dp, ip]
-    L.scadyn_tjob_create.argtypes = [P, ctypes.c_int32, ctypes.c_int32, ip, ctypes.POINTER(P)]
+    L.scadyn_tjob_create.argtypes = [P, ctypes.c_int32, ctypes.c_int32, ip, ctypes.c_int32, ctypes.POINTER(P)]
     L.scadyn_tjob_advance.argtypes = [P, ctypes.c_int32, ip]
     L.scadyn_tjob_result.argtypes = [P, dp, ip]
     L.scadyn_tjob_destroy.argtypes = [P]
@@ -102,6 +102,7 @@ def lib():
                                        ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]
     L.scadyn_server_status.argtypes = [P, ctypes.c_int64, ip, dp]
     L.scadyn_server_wait.argtypes = [P, ctypes.c_int64, dp, dp, dp, dp, ip, dp]
+    L.scadyn_server_cancel.argtypes = [P, ctypes.c_int64]
     L.scadyn_server_destroy.argtypes = [P]
     L.scadyn_server_destroy.restype = None
     for name in EXPORTED:
@@ -274,12 +275,12 @@ class Operator:
 class TJob:
     """Resumable T-matrix solve over a column list (scadyn_tjob_*)."""
 
-    def __init__(self, op, nmax, cols):
+    def __init__(self, op, nmax, cols, spread=0):
         self.op = op  # keeps the operator alive
         self.nmax = nmax
         self.cols = np.ascontiguousarray(np.asarray(cols), dtype=np.int32)
         h = ctypes.c_void_p()
-        check(lib().scadyn_tjob_create(op.h, nmax, self.cols.size, _ip(self.cols), ctypes.byref(h)))
+        check(lib().scadyn_tjob_create(op.h, nmax, self.cols.size, _ip(self.cols), int(spread), ctypes.byref(h)))
         self.h = h
 
     def advance(self, want):
@@ -360,6 +361,9 @@ class Server:
         t = np.zeros(4)
         check(lib().scadyn_server_wait(self.h, rid, *[_dp(x.view(np.float64)) for x in T], _ip(it), _dp(t)))
         return tuple(x.T.copy() for x in T), it, t
+
+    def cancel(self, rid):
+        check(lib().scadyn_server_cancel(self.h, rid))
 
     def close(self):
         if getattr(self, 'h', None):

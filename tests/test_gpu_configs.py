@@ -54,3 +54,148 @@ def test_sphere43k_tmatrix_vs_mie():
     assert ea < 0.10 and eb < 0.10
     assert off < 0.05
     assert rec < 1e-3
+
+
+def _col(n, m, nm, wave):
+    """0-based T-matrix column of VSWF (n, m), wave 0 = M, 1 = N."""
+    return wave * nm + n * (n + 1) + m - 1
+
+
+def _matvec_vs_port(c, e, param, k, tol64, tol32):
+    """HIP matvec (FP64 and FP32 grids) on two slots against the CPU port of the
+    reference matvec on the same operator arrays (the port equals the oracle's
+    compute_Ax2 to 1e-12, test_cpu_port.py)."""
+    from oracle import cpu_port
+    from scadyn_amd import native
+    op = native.Operator(c, e, param, k, batch=2)
+    port = cpu_port.PortOperator(c, e, k, op.export('S'), op.export('blocks'), op.export('blkind'))
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((2, 3 * e.shape[0])) + 1j * rng.standard_normal((2, 3 * e.shape[0]))
+    y64 = op.matvec(x)
+    y32 = op.matvec(x, precision=native.PREC_MIXED)
+    op.close()
+    for b in range(2):
+        yp = port.matvec(x[b], cpu_port.lib().port_threads())
+        d64, d32 = rel(y64[b], yp), rel(y32[b], yp)
+        print(f'[gpu-configs]   slot {b}: FP64 matvec {d64:.2e}, FP32 matvec {d32:.2e} from the port', flush=True)
+        assert d64 < tol64 and d32 < tol32
+
+
+def _mixed_vs_fp64_columns(c, e, param, k, nmax, cols, tol):
+    """T columns of the mixed solve against tightly converged FP64 ones."""
+    from scadyn_amd import native
+    out = {}
+    for name, kw in (('fp64', dict(tol=tol)),
+                     ('mixed', dict(precision=native.PREC_MIXED, tol=1e-4, refine_tol=tol))):
+        op = native.Operator(c, e, param, k, batch=len(cols), restart=60, **kw)
+        out[name] = op.tmatrix_cols(nmax, cols)
+        op.close()
+        print(f'[gpu-configs]   {name}: iterations {out[name][1].tolist()}', flush=True)
+    T64, it64 = out['fp64']
+    T32, it32 = out['mixed']
+    assert it64.max() < 50 * 60 and it32.max() < 50 * 60       # converged, not capped
+    return T64, T32
+
+
+def test_ellipsoid200k_matvec_mixed_and_reciprocity():
+    from scadyn_amd.workloads import make_workload
+    c, e, param, k, nmax, desc = make_workload('ellipsoid200k')
+    print(f'[gpu-configs] ellipsoid200k: {e.shape[0]} tets, nmax {nmax}', flush=True)
+    _matvec_vs_port(c, e, param, k, 1e-9, 2e-5)
+    nm = (nmax + 1) ** 2 - 1
+    # columns closed under m -> -m, so that T_ij = T_{jbar ibar} can be checked within them
+    pairs = [(1, 1, 0), (1, -1, 0), (2, 1, 1), (2, -1, 1)]
+    cols = [_col(n, m, nm, w) for n, m, w in pairs]
+    T64, T32 = _mixed_vs_fp64_columns(c, e, param, k, nmax, cols, 1e-10)
+    err = rel(T32, T64)
+    print(f'[gpu-configs]   mixed vs FP64 T columns: {err:.2e}', flush=True)
+    assert err < 1e-6
+    bar = {cols[0]: cols[1], cols[1]: cols[0], cols[2]: cols[3], cols[3]: cols[2]}
+    pos = {cc: j for j, cc in enumerate(cols)}
+    worst = 0.0
+    for i in cols:
+        for j in cols:
+            a, b = T64[i, pos[j]], T64[bar[j], pos[bar[i]]]
+            worst = max(worst, abs(a - b) / np.abs(T64).max())
+    print(f'[gpu-configs]   reciprocity defect {worst:.2e}', flush=True)
+    assert worst < 1e-4
+
+
+def test_layered824k_matvec_and_mixed_columns():
+    from scadyn_amd.workloads import make_workload
+    c, e, param, k, nmax, desc = make_workload('layered824k')
+    print(f'[gpu-configs] layered824k: {e.shape[0]} tets, nmax {nmax}', flush=True)
+    _matvec_vs_port(c, e, param, k, 1e-9, 2e-5)
+    nm = (nmax + 1) ** 2 - 1
+    cols = [_col(1, 0, nm, 1), _col(20, 7, nm, 0)]
+    T64, T32 = _mixed_vs_fp64_columns(c, e, param, k, nmax, cols, 1e-9)
+    err = rel(T32, T64)
+    print(f'[gpu-configs]   mixed vs FP64 T columns: {err:.2e}', flush=True)
+    assert err < 1e-6
+
+
+def test_wavelength_bars_alone_equal_mix():
+    """One random ellipsoid at several size parameters (its own grid and
+    expansion order per bar, as the 16-bar workload) through the request loop
+    in mixed precision: each bar's T is the same alone and in the mix."""
+    from scadyn_amd import native
+    from scadyn_amd.workloads import A_RADIUS, ellipsoid_mesh
+    c, e = ellipsoid_mesh(20000, 0)
+    p = np.full(e.shape[0], 3.0 + 0.1j)
+    bars = [(2.0, 3), (4.0, 4), (7.0, 3)]
+    kw = dict(precision=native.PREC_MIXED, tol=1e-4, refine_tol=1e-8, restart=40)
+    print(f'[gpu-configs] bars: {e.shape[0]} tets, x {[b[0] for b in bars]}', flush=True)
+    alone = []
+    for x, nmax in bars:
+        op = native.Operator(c, e, p, x / A_RADIUS, batch=min(2 * ((nmax + 1) ** 2 - 1), 32), **kw)
+        alone.append(op.tmatrix(nmax)[0])
+        op.close()
+    sv = native.Server(max_active=3)
+    ids = [sv.submit(c, e, p, x / A_RADIUS, nmax, **kw) for x, nmax in bars]
+    mix = [sv.wait(i)[0] for i in ids]
+    sv.close()
+    for Ta, Tm in zip(alone, mix):
+        for a, b in zip(Tm, Ta):
+            assert rel(a, b) < 1e-13
+
+
+def test_serving_admission_of_a_400k_tet_request():
+    """A request at the top of the serving range (about 400k tets, x 15,
+    nmax 27, mixed): admitted by footprint under the default budget and set up
+    on the device, then cancelled; a budget below its footprint rejects it at
+    submission; a budget of 1.5 footprints admits one and queues the next; the
+    loop still serves a small request afterwards."""
+    import time
+    from scadyn_amd import native
+    from scadyn_amd.workloads import A_RADIUS, ellipsoid_mesh, nmax_for
+    c, e = ellipsoid_mesh(400000, 7)
+    p = np.full(e.shape[0], 2.5 + 0.1j)
+    k, nmax = 15.0 / A_RADIUS, nmax_for(15.0)
+    kw = dict(precision=native.PREC_MIXED, tol=1e-4, refine_tol=1e-8)
+    fp = native.footprint(c, e, p, nmax, batch=32, **kw)
+    print(f'[gpu-configs] serving: {e.shape[0]} tets, nmax {nmax}, footprint {fp / 1e9:.1f} GB', flush=True)
+    with pytest.raises(native.ScadynError) as ex:
+        sv = native.Server(hbm_budget=fp - 1)
+        try:
+            sv.submit(c, e, p, k, nmax, **kw)
+        finally:
+            sv.close()
+    assert ex.value.code == native.ERR_MEMORY
+    sv = native.Server(hbm_budget=int(1.5 * fp))
+    a = sv.submit(c, e, p, k, nmax, **kw)
+    b = sv.submit(c, e, p, k, nmax, **kw)
+    t0 = time.time()
+    while sv.status(a)[1][2] == 0.0:          # admitted and set up on the device
+        assert time.time() - t0 < 120
+        time.sleep(0.2)
+    assert sv.status(a)[0] == native.REQ_RUNNING and sv.status(b)[0] == native.REQ_QUEUED
+    print(f'[gpu-configs]   set up in {sv.status(a)[1][2] - sv.status(a)[1][1]:.1f} s; cancelling', flush=True)
+    for rid in (b, a):
+        sv.cancel(rid)
+        with pytest.raises(native.ScadynError, match='cancelled'):
+            sv.wait(rid)
+    cs, es = ellipsoid_mesh(5000, 3)
+    small = sv.submit(cs, es, np.full(es.shape[0], 2.0 + 0.0j), 2.0 / A_RADIUS, 2, **kw)
+    T, it, times = sv.wait(small)
+    sv.close()
+    assert np.linalg.norm(T[0]) > 0 and (it > 0).all()

@@ -70,3 +70,22 @@ def test_host_only_loop_admits_and_retires_concurrently():
     sv.close()
     assert not errors, errors
     assert not any(t.is_alive() for t in threads)
+
+
+def test_host_only_cancel_queued_and_running():
+    from scadyn_amd import native
+    c, e, p, k = sphere_case()
+    fp = native.footprint(c, e, p, 1)
+    sv = native.Server(max_active=1, hbm_budget=int(1.5 * fp), host_only=True)
+    a = sv.submit(c, e, p, k, 1)
+    b = sv.submit(c, e, p, k, 1)
+    sv.cancel(b)                                   # queued behind a: leaves the queue
+    with pytest.raises(native.ScadynError, match='cancelled'):
+        sv.wait(b)
+    T, it, _ = sv.wait(a)                          # a is unaffected
+    assert (it == a).all()
+    d = sv.submit(c, e, p, k, 1)
+    sv.cancel(d)
+    with pytest.raises(native.ScadynError, match='cancelled'):
+        sv.wait(d)
+    sv.close()
