@@ -797,6 +797,14 @@ struct SolveSpec {
     // matvec and a system is done only when it is <= tol_true ||b||
     bool verify64 = false;
     double tol_true = 0.0;
+    // aligned batch: a slot whose restart cycle ends waits (PH_PARK) until no
+    // slot of the batch is in its Arnoldi phase; then every waiting slot
+    // restarts and free slots are refilled together.  All slots then share
+    // one Arnoldi index, so a step's orthogonalisation passes and the mixed
+    // solve's FP64 restart matvec serve the whole batch at once, instead of
+    // every step running the longest slot's passes for half-idle lanes and a
+    // narrow restart matvec.  Per-slot arithmetic is unchanged.
+    bool align = true;
 };
 
 // Pinned host buffer (page-locked, so async copies never read a reused buffer).
@@ -890,8 +898,10 @@ struct ColumnSolver {
             HIP_CHECK(hipMemsetAsync(w->V, 0, sizeof(cd) * (size_t)(w->m + 2) * nB, op->s));
         }
         // every system finishes within restart cycles of at most m+1 matvecs; bound the loop
-        max_steps = ((long long)spec.restart * (w->m + 1) + 4) * (((long long)cols.size() + B - 1) / B + 2) + 16 +
-                    spread;
+        // (an aligned batch adds at most two waiting steps per cycle and one
+        // cycle's wait before a refill)
+        max_steps = ((long long)spec.restart * (w->m + 3) + w->m + 8) * (((long long)cols.size() + B - 1) / B + 2) +
+                    16 + spread;
         HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     }
 
@@ -905,11 +915,30 @@ struct ColumnSolver {
     void launch() {
         if (++steps > max_steps) fail(SCADYN_ERR_DEVICE, "GMRES driver made no progress");
         hipStream_t s = op->s;
-        // refill free slots
+        // cycle boundary of an aligned batch: no slot in its Arnoldi phase.
+        // Free slots are refilled up to two steps into a cycle (a newcomer
+        // lags the batch by at most two steps, which the others wait for at
+        // the cycle's end), so a system that finishes at a restart is
+        // replaced at once rather than a cycle later.
+        int maxi = -1;
+        for (int b = 0; b < B; ++b)
+            if (slot_col[b] >= 0 && hs.p[b].phase == PH_ARNOLDI) maxi = std::max(maxi, hs.p[b].i);
+        const bool boundary = !spec.align || maxi < 0;
+        const bool refill_ok = !spec.align || maxi < 2;
+        if (spec.align && boundary) {
+            bool any_park = false;
+            for (int b = 0; b < B; ++b)
+                if (slot_col[b] >= 0 && hs.p[b].phase == PH_PARK) {
+                    hs.p[b].phase = PH_RESTART;
+                    any_park = true;
+                }
+            if (any_park) launch_unpark(d, w->st, s);
+        }
+        // refill free slots (at a boundary when aligned)
         bool any_load = false;
         for (int b = 0; b < B; ++b) {
             h_load.p[b] = -1;
-            if (slot_col[b] < 0 && next < (int)cols.size() && steps - 1 >= (long long)b * spread / B) {
+            if (refill_ok && slot_col[b] < 0 && next < (int)cols.size() && steps - 1 >= (long long)b * spread / B) {
                 h_load.p[b] = cols[next++];
                 slot_col[b] = h_load.p[b];
                 any_load = true;
@@ -988,7 +1017,7 @@ struct ColumnSolver {
                 }
             }
             launch_slot_update(d, w->st, w->H, w->cs, w->sn, w->g, w->y, w->nrm2, w->nav2, spec.tol, spec.restart,
-                               spec.verify64 ? 1 : 0, spec.tol_true, s);
+                               spec.verify64 ? 1 : 0, spec.tol_true, spec.align ? 1 : 0, s);
             if (w->Vf)
                 launch_vec_update(d, w->st, (const cf *)w->Wf, w->Vf, w->X, w->y, s);
             else

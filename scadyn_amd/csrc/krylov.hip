@@ -475,7 +475,7 @@ __global__ void k_slot_update(KrylovDims d, SlotState *__restrict__ st, cd *__re
                               cd *__restrict__ csall, cd *__restrict__ snall, cd *__restrict__ gall,
                               cd *__restrict__ yall, const double *__restrict__ nrm2,
                               const double *__restrict__ nav2, double tol, int restart, int verify,
-                              double tol_true) {
+                              double tol_true, int park) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= d.B) return;
     SlotState s = st[b];
@@ -535,7 +535,7 @@ __global__ void k_slot_update(KrylovDims d, SlotState *__restrict__ st, cd *__re
             len = i + 1;
             if (verify) {
                 s.cycle++;
-                s.phase = (s.cycle >= restart) ? PH_DONE : PH_RESTART;
+                s.phase = (s.cycle >= restart) ? PH_DONE : (park ? PH_PARK : PH_RESTART);
             } else {
                 s.conv = 1;
                 s.phase = PH_DONE;
@@ -545,7 +545,7 @@ __global__ void k_slot_update(KrylovDims d, SlotState *__restrict__ st, cd *__re
             if (i + 1 == m) {
                 len = m;
                 s.cycle++;
-                s.phase = (s.cycle >= restart) ? PH_DONE : PH_RESTART;
+                s.phase = (s.cycle >= restart) ? PH_DONE : (park ? PH_PARK : PH_RESTART);
             } else {
                 s.act = ACT_NEXT;
                 s.tgt = i + 1;
@@ -706,6 +706,11 @@ __global__ void k_set_inputs(KrylovDims d, const SlotState *__restrict__ st, lon
 }
 
 // slots whose column was stored become idle
+__global__ void k_unpark(KrylovDims d, SlotState *__restrict__ st) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < d.B && st[b].phase == PH_PARK) st[b].phase = PH_RESTART;
+}
+
 __global__ void k_retire(KrylovDims d, const int *__restrict__ col, SlotState *__restrict__ st) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= d.B || col[b] < 0) return;
@@ -821,9 +826,13 @@ void launch_mgs_reduce(const KrylovDims &d, int k, const SlotState *st, cd *H, c
 }
 void launch_slot_update(const KrylovDims &d, SlotState *st, cd *H, cd *cs, cd *sn, cd *g, cd *y,
                         const double *nrm2, const double *nav2, double tol, int restart, int verify, double tol_true,
-                        hipStream_t s) {
+                        int park, hipStream_t s) {
     hipLaunchKernelGGL(k_slot_update, dim3((d.B + 63) / 64), dim3(64), 0, s, d, st, H, cs, sn, g, y, nrm2, nav2,
-                       tol, restart, verify, tol_true);
+                       tol, restart, verify, tol_true, park);
+    HIP_CHECK(hipGetLastError());
+}
+void launch_unpark(const KrylovDims &d, SlotState *st, hipStream_t s) {
+    hipLaunchKernelGGL(k_unpark, dim3((d.B + 63) / 64), dim3(64), 0, s, d, st);
     HIP_CHECK(hipGetLastError());
 }
 template <typename VT, typename WT>

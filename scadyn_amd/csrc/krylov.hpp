@@ -9,7 +9,9 @@
 
 namespace scadyn {
 
-enum SlotPhase : int { PH_IDLE = 0, PH_RESTART = 1, PH_ARNOLDI = 2, PH_DONE = 3 };
+// PH_PARK: the slot's restart cycle has ended and it waits for the batch's
+// next cycle boundary (aligned batches); every kernel treats it as idle.
+enum SlotPhase : int { PH_IDLE = 0, PH_RESTART = 1, PH_ARNOLDI = 2, PH_DONE = 3, PH_PARK = 4 };
 enum SlotAction : int { ACT_NONE = 0, ACT_V0 = 1, ACT_NEXT = 2, ACT_XUPD = 3 };
 
 struct SlotState {
@@ -42,9 +44,12 @@ void launch_mgs_reduce(const KrylovDims &d, int k, const SlotState *st, cd *H, c
 // verify = 1: restart residuals are true FP64 residuals and decide convergence
 // against tol_true (mixed precision); an Arnoldi estimate below tol times the
 // cycle's starting residual ends a cycle
+// park = 1: a slot whose cycle ends goes to PH_PARK instead of PH_RESTART
 void launch_slot_update(const KrylovDims &d, SlotState *st, cd *H, cd *cs, cd *sn, cd *g, cd *y,
                         const double *nrm2, const double *nav2, double tol, int restart, int verify, double tol_true,
-                        hipStream_t s);
+                        int park, hipStream_t s);
+// PH_PARK -> PH_RESTART for every slot (a cycle boundary of the batch)
+void launch_unpark(const KrylovDims &d, SlotState *st, hipStream_t s);
 // basis V stored as complex128 or (mixed precision) complex64; x in X (n x B)
 void launch_vec_update(const KrylovDims &d, const SlotState *st, const cd *W, cd *V, cd *X, const cd *y,
                        hipStream_t s);
