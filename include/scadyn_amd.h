@@ -166,6 +166,19 @@ int scadyn_op_vswf(scadyn_operator *op, int32_t nmax, double *mat);
  * FP64 matrix-core GEMM of scadyn_op_tmatrix. */
 int scadyn_op_project(scadyn_operator *op, int32_t nmax, const double *X, double *T);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the projection GEMM of scadyn_op_project on device
+ * pointers, T = mat^H X with mat (n x na) and X (n x nx) row-major complex128
+ * (any row order, the same in both) and T (na x nx) row-major, on the
+ * operator's device and stream, synchronous.  The GEMM's row-chunk partials
+ * live in a fixed per-device scratch array of 4 Mi complex128 elements
+ * (64 MiB, held by each device the library runs on until the process ends):
+ * na * nx above that returns SCADYN_ERR_ARG.  Nothing is allocated.  Calls
+ * that overlap on one device share the scratch and race. */
+int scadyn_op_project_noalloc(scadyn_operator *op, int64_t n, int32_t na, int32_t nx, const double *mat,
+                              const double *X, double *T);
+
 /* T-matrix blocks of this operator (compute_T_matrix, src/T_matrix.f90:240-277). */
 int scadyn_op_tmatrix(scadyn_operator *op, int32_t nmax,
                       double *Taa, double *Tab, double *Tba, double *Tbb,

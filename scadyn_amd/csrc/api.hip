@@ -1497,6 +1497,36 @@ void scadyn_server::run() {
 namespace scadyn {
 }  // namespace scadyn
 
+// ================================================================ no-allocation projection
+// Row-chunk partials of the projection GEMM for scadyn_op_project_noalloc: a
+// file-scope device array (each device holds its own copy for the life of the
+// process), so that entry point calls no allocator.  Host code must not pass
+// the array's name to a kernel: it is a host-side stand-in there.  The device
+// address is taken once per device with that device current and kept in
+// g_scratch_ptr.
+namespace scadyn {
+constexpr size_t NOALLOC_SCRATCH_ELEMS = (size_t)4 << 20;  // complex128 elements: 64 MiB
+static __device__ __align__(256) double scratch[2 * NOALLOC_SCRATCH_ELEMS];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static cd *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {nullptr};
+
+// the device address of `scratch` on `device` (current), looked up on first use
+static cd *scratch_ptr(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) fail(SCADYN_ERR_ARG, "device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            fail(SCADYN_ERR_DEVICE, std::string("hipGetSymbolAddress of the projection scratch failed: ") +
+                                        hipGetErrorString(e));
+        g_scratch_ptr[device] = (cd *)p;
+    }
+    return g_scratch_ptr[device];
+}
+}  // namespace scadyn
+
 // ================================================================ C ABI
 #define ABI_TRY try {
 #define ABI_CATCH                                                   \
@@ -1751,6 +1781,23 @@ int scadyn_op_project(scadyn_operator *op, int32_t nmax, const double *X, double
     cd *o = (cd *)T;
     for (int i = 0; i < ncol; ++i)
         for (int j = 0; j < ncol; ++j) o[(size_t)j * ncol + i] = hT[(size_t)i * ncol + j];
+    ABI_CATCH
+}
+
+int scadyn_op_project_noalloc(scadyn_operator *op, int64_t n, int32_t na, int32_t nx, const double *mat,
+                              const double *X, double *T) {
+    ABI_TRY
+    if (!op || !mat || !X || !T) fail(SCADYN_ERR_ARG, "null argument");
+    if (n < 1 || na < 1 || nx < 1) fail(SCADYN_ERR_ARG, "n, na and nx must be >= 1");
+    if ((size_t)na * (size_t)nx > NOALLOC_SCRATCH_ELEMS)
+        fail(SCADYN_ERR_ARG, "na * nx = " + std::to_string((size_t)na * (size_t)nx) +
+                                 " exceeds the no-allocation limit of " + std::to_string(NOALLOC_SCRATCH_ELEMS) +
+                                 " elements");
+    HIP_CHECK(hipSetDevice(op->device));
+    cd *work = scratch_ptr(op->device);
+    // every partial the reduction reads is written by the GEMM first: nothing to clear
+    launch_tgemm((const cd *)mat, (const cd *)X, (long long)n, na, nx, (cd *)T, work, NOALLOC_SCRATCH_ELEMS, op->s);
+    HIP_CHECK(hipStreamSynchronize(op->s));
     ABI_CATCH
 }
 

@@ -20,7 +20,8 @@ KCLASS = ['scatter', 'fftx', 'ffty', 'fftz_conv', 'iffty', 'ifftx', 'gather_near
 # every symbol include/scadyn_amd.h declares
 EXPORTED = ['scadyn_opts_default', 'scadyn_last_error', 'scadyn_tmat_solve', 'scadyn_op_create',
             'scadyn_op_destroy', 'scadyn_op_info', 'scadyn_op_export', 'scadyn_op_matvec', 'scadyn_op_matvec_prec',
-            'scadyn_op_solve', 'scadyn_op_vswf', 'scadyn_op_project', 'scadyn_op_tmatrix', 'scadyn_op_tmatrix_cols',
+            'scadyn_op_solve', 'scadyn_op_vswf', 'scadyn_op_project', 'scadyn_op_project_noalloc',
+            'scadyn_op_tmatrix', 'scadyn_op_tmatrix_cols',
             'scadyn_tjob_create', 'scadyn_tjob_advance', 'scadyn_tjob_result', 'scadyn_tjob_destroy',
             'scadyn_op_profile',
             'scadyn_op_timings', 'scadyn_op_timings_prec', 'scadyn_footprint', 'scadyn_server_create', 'scadyn_server_submit',
@@ -86,6 +87,7 @@ def lib():
     L.scadyn_op_vswf.argtypes = [P, ctypes.c_int32, dp]
     L.scadyn_op_project.argtypes = [P, ctypes.c_int32, dp, dp]
     L.scadyn_op_tmatrix.argtypes = [P, ctypes.c_int32, dp, dp, dp, dp, ip]
+    L.scadyn_op_project_noalloc.argtypes = [P, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, P, P, P]
     L.scadyn_op_tmatrix_cols.argtypes = [P, ctypes.c_int32, ctypes.c_int32, ip, dp, ip]
     L.scadyn_tjob_create.argtypes = [P, ctypes.c_int32, ctypes.c_int32, ip, ctypes.c_int32, ctypes.POINTER(P)]
     L.scadyn_tjob_advance.argtypes = [P, ctypes.c_int32, ip]
@@ -239,6 +241,11 @@ class Operator:
         T = np.zeros((2 * nm, 2 * nm), np.complex128)  # filled column-major
         check(lib().scadyn_op_project(self.h, nmax, _dp(Xc.view(np.float64)), _dp(T.view(np.float64))))
         return T.T.copy()
+
+    def project_noalloc(self, n, na, nx, mat_ptr, X_ptr, T_ptr):
+        """T = mat^H X on device pointers (row-major complex128), no allocation."""
+        check(lib().scadyn_op_project_noalloc(self.h, int(n), int(na), int(nx), ctypes.c_void_p(mat_ptr),
+                                              ctypes.c_void_p(X_ptr), ctypes.c_void_p(T_ptr)))
 
     def tmatrix(self, nmax):
         nm = (nmax + 1) ** 2 - 1
