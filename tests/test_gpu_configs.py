@@ -50,10 +50,12 @@ def test_sphere43k_tmatrix_vs_mie():
     rec = _reciprocity_defect(T, nmax)
     print(f'[gpu-configs] sphere43k vs Mie: Taa {ea:.4f} Tbb {eb:.4f} off-diagonal {off:.2e} '
           f'reciprocity {rec:.2e}; iterations {it.min()}-{it.max()}', flush=True)
+    # measured: Taa 0.94%, Tbb 2.34% (the pFFT far-field level), off-diagonal 1.6e-3,
+    # reciprocity 3.6e-6 (tol 1e-5 solves)
     assert it.max() < 200                    # every column converged to tol 1e-5
-    assert ea < 0.10 and eb < 0.10
-    assert off < 0.05
-    assert rec < 1e-3
+    assert ea < 0.02 and eb < 0.035
+    assert off < 5e-3
+    assert rec < 1e-4
 
 
 def _col(n, m, nm, wave):
