@@ -42,6 +42,30 @@ def build_fortran(verbose=False):
     return FORTRAN_EXE
 
 
+HDF5_PREFIX = '/opt/conda'
+H5_LIB = os.path.join(HERE, 'libscadyn_h5.so')
+
+
+def build_h5(verbose=False):
+    """libscadyn_h5.so: the HDF5 mesh / T-matrix file formats (host C, HDF5 C API),
+    a library of its own so that the solver library does not depend on HDF5."""
+    inc = os.path.join(HDF5_PREFIX, 'include')
+    libdir = os.path.join(HDF5_PREFIX, 'lib')
+    if not os.path.exists(os.path.join(inc, 'hdf5.h')):
+        return None
+    tmp = H5_LIB + '.tmp'
+    cmd = ['gcc', '-O2', '-std=c11', '-D_DEFAULT_SOURCE', '-fPIC', '-shared', '-Wall', '-I' + inc, '-o', tmp,
+           os.path.join(CSRC, 'h5io.c'), '-L' + libdir, '-lhdf5', '-Wl,-rpath,' + libdir]
+    if verbose:
+        print(' '.join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError('building libscadyn_h5.so failed')
+    os.replace(tmp, H5_LIB)
+    return H5_LIB
+
+
 def build(verbose=False):
     hipcc = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
     os.makedirs(OBJDIR, exist_ok=True)
@@ -72,6 +96,7 @@ def build(verbose=False):
         raise RuntimeError('linking libscadyn_amd.so failed')
     os.replace(tmp, LIB)
     build_fortran(verbose)
+    build_h5(verbose)
     return LIB
 
 
