@@ -234,6 +234,24 @@ int scadyn_op_timings(scadyn_operator *op, double *timings /* [COUNT][3] */);
  * reported whole. */
 int scadyn_op_timings_prec(scadyn_operator *op, int32_t precision, double *timings /* [COUNT][3] */);
 
+/* ---- consumers of a T-matrix (host) ------------------------------------ */
+
+/* Orientation-averaged cross sections of a bar's T (tr_T, src/mie.f90:728-759):
+ * crs = {C_ext, C_sca, C_abs, Tr(Re T + T T^H)}, from the four nm x nm blocks
+ * (column-major, as scadyn_tmat_solve returns them) and the wavenumber k. */
+int scadyn_cross_sections(int32_t nmax, const double *Taa, const double *Tab, const double *Tba, const double *Tbb,
+                          double k, double crs[4]);
+
+/* Radiation force and torque along the incident direction for an x-polarised
+ * plane wave of amplitude E (V/m) travelling along +z, the particle in its
+ * reference orientation (forcetorque's z components, src/forces.f90:81-189,
+ * F_z / T_z :215-257, planewave src/mie.f90:204-228): out = {F_z (N),
+ * T_z (N m), Q_f,z, Q_t,z}, the efficiencies normalised as get_forces
+ * (src/forces.f90:55-63) with the radius a (m).  The transverse components
+ * need the VSWF rotations of the dynamics code and are not provided. */
+int scadyn_force_torque_z(int32_t nmax, const double *Taa, const double *Tab, const double *Tba, const double *Tbb,
+                          double k, double E, double a, double out[4]);
+
 /* ---- request loop (serving) ------------------------------------------ */
 
 /* Device bytes one T-matrix request needs at its peak (operator + the larger of

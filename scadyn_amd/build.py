@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libscadyn_amd.so')
-SOURCES = ['api.hip', 'geom.hip', 'setup.hip', 'fft.hip', 'fft_reg.hip', 'matvec.hip', 'krylov.hip']
+SOURCES = ['api.hip', 'geom.hip', 'setup.hip', 'fft.hip', 'fft_reg.hip', 'matvec.hip', 'krylov.hip', 'consumer.hip']
 OBJDIR = os.path.join(HERE, '_build')
 
 HIPCC_FLAGS = [

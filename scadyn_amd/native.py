@@ -24,7 +24,8 @@ EXPORTED = ['scadyn_opts_default', 'scadyn_last_error', 'scadyn_tmat_solve', 'sc
             'scadyn_op_tmatrix', 'scadyn_op_tmatrix_cols',
             'scadyn_tjob_create', 'scadyn_tjob_advance', 'scadyn_tjob_result', 'scadyn_tjob_destroy',
             'scadyn_op_profile',
-            'scadyn_op_timings', 'scadyn_op_timings_prec', 'scadyn_footprint', 'scadyn_server_create', 'scadyn_server_submit',
+            'scadyn_op_timings', 'scadyn_op_timings_prec', 'scadyn_cross_sections', 'scadyn_force_torque_z',
+            'scadyn_footprint', 'scadyn_server_create', 'scadyn_server_submit',
             'scadyn_server_status', 'scadyn_server_wait', 'scadyn_server_cancel', 'scadyn_server_destroy']
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 
@@ -97,6 +98,9 @@ def lib():
     L.scadyn_op_profile.argtypes = [P, ctypes.c_int32]
     L.scadyn_op_timings.argtypes = [P, dp]
     L.scadyn_op_timings_prec.argtypes = [P, ctypes.c_int32, dp]
+    L.scadyn_cross_sections.argtypes = [ctypes.c_int32, dp, dp, dp, dp, ctypes.c_double, dp]
+    L.scadyn_force_torque_z.argtypes = [ctypes.c_int32, dp, dp, dp, dp, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, dp]
     L.scadyn_footprint.argtypes = [ctypes.POINTER(CMesh), ctypes.POINTER(COpts), ctypes.c_int32,
                                    ctypes.POINTER(ctypes.c_int64)]
     L.scadyn_server_create.argtypes = [ctypes.POINTER(CServerOpts), ctypes.POINTER(P)]
@@ -324,6 +328,31 @@ def tmat_solve(coord, etopol, param, k, nmax, **opts):
     check(lib().scadyn_tmat_solve(ctypes.byref(m.c), ctypes.byref(o), float(k), nmax,
                                   *[_dp(t.view(np.float64)) for t in T], _ip(it)))
     return tuple(t.T.copy() for t in T), it
+
+
+def _blocks(T):
+    nm = T[0].shape[0]
+    nmax = int(round((nm + 1) ** 0.5)) - 1
+    cols = [np.asfortranarray(np.asarray(t, np.complex128)) for t in T]
+    return nmax, cols
+
+
+def cross_sections(T, k):
+    """Orientation-averaged (C_ext, C_sca, C_abs, Tr) of T = (Taa, Tab, Tba, Tbb)."""
+    nmax, cols = _blocks(T)
+    out = np.zeros(4)
+    check(lib().scadyn_cross_sections(nmax, *[c.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for c in cols],
+                                      float(k), _dp(out)))
+    return out
+
+
+def force_torque_z(T, k, E, a):
+    """(F_z, T_z, Q_f_z, Q_t_z) for an x-polarised plane wave along +z."""
+    nmax, cols = _blocks(T)
+    out = np.zeros(4)
+    check(lib().scadyn_force_torque_z(nmax, *[c.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for c in cols],
+                                      float(k), float(E), float(a), _dp(out)))
+    return out
 
 
 def footprint(coord, etopol, param, nmax, **opts):

@@ -56,6 +56,17 @@ def test_sphere43k_tmatrix_vs_mie():
     assert ea < 0.02 and eb < 0.035
     assert off < 5e-3
     assert rec < 1e-4
+    # the first consumers of T: cross sections and radiation pressure vs Mie (x = 2)
+    from oracle import mie
+    from scadyn_amd.workloads import A_RADIUS
+    crs = native.cross_sections(T, k)
+    qext, qsca, _ = mie.efficiencies(2.0, np.sqrt(2.5 + 0.01j), 12)
+    F, Tz, Qf, Qt = native.force_torque_z(T, k, 1e4, A_RADIUS)
+    area = np.pi * A_RADIUS ** 2
+    print(f'[gpu-configs]   Q_ext {crs[0] / area:.5f} (Mie {qext:.5f}), Q_sca {crs[1] / area:.5f} '
+          f'(Mie {qsca:.5f}), Q_pr,z {Qf:.5f}', flush=True)
+    assert abs(crs[0] / area - qext) < 0.03 * qext and abs(crs[1] / area - qsca) < 0.03 * qsca
+    assert 0 < Qf < crs[0] / area
 
 
 def _col(n, m, nm, wave):
