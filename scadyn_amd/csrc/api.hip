@@ -764,7 +764,7 @@ static int auto_batch(scadyn_operator *op, int ncols) {
     size_t free_b = 0, total_b = 0;
     HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
     size_t per = slot_bytes(op, op->opts.maxit);
-    size_t cap = (size_t)(0.6 * (double)free_b) / std::max<size_t>(per, 1);
+    size_t cap = (size_t)(0.75 * (double)free_b) / std::max<size_t>(per, 1);
     if (cap == 0) fail(SCADYN_ERR_MEMORY, "not enough device memory for one RHS slot");
     // Up to 64 slots; above 64 columns, split them into the fewest equal
     // generations of at most 128 slots that fill their last 64-slot chunk to at

@@ -252,5 +252,55 @@ GC_HD void for_near_cubes(const GatherArgs<C> &g, const Item &it, F &&f) {
     }
 }
 
+// ---- near zone on the matrix cores (complex64 matvec) ----------------------
+// For one chunk of nv column tets, the near-zone product of the pass's NT row
+// tets is the real matrix product D = Am Bm over 64 slots:
+//   D[n'][s]  rows n' = 2 (3u + i) + co: row tet u, component i, co = re / im
+//   Bm[k'][s] = ci-part (re / im) of x_{3v+j}[s], k' = 2 (3v + j) + ci
+//   Am[n'][k'] = Re A_uv(i, j) if co == ci, -Im A_uv(i, j) if (co, ci) = (re, im),
+//                +Im A_uv(i, j) if (co, ci) = (im, re)
+// (y_re += a x_re - b x_im, y_im += b x_re + a x_im).  One
+// v_mfma_f32_16x16x4f32 per (k-step of 4, row tile of 16, slot tile of 16):
+// lane l holds Am[16 rt + (l & 15)][4 s + (l >> 4)] and
+// Bm[4 s + (l >> 4)][16 st + (l & 15)], and output element e of row
+// mf_row(l, e), column l & 15 of its tile.  After the chunks, the tiles go
+// through an LDS exchange buffer [16 RT][64] to the lanes that own the slots.
+GC_HD constexpr int mf_rows(int NT) { return (6 * NT + 15) / 16; }  // row tiles of the pass
+constexpr int MF_XBYTES = 16 * 3 * 64 * (int)sizeof(float);           // exchange buffer, NT <= 8
+GC_HD int mf_row(int lane, int e) { return 4 * (lane >> 4) + e; }
+
+// Am[np][kp] from the staged blocks (zero outside the pass's rows and the chunk)
+GC_HD float mf_a(const cf *smb, int NT, int nv, int VC, int np, int kp) {
+    const int u = np / 6, v = kp / 6;
+    if (u >= NT || v >= nv) return 0.0f;
+    const int i = (np % 6) >> 1, co = np & 1, j = (kp % 6) >> 1, ci = kp & 1;
+    const cf a = smb[(u * VC + v) * 9 + i + 3 * j];
+    return co == ci ? a.x : (co ? a.y : -a.y);
+}
+
+// Bm[kp][slot] (zero beyond the chunk or for a slot past the batch)
+GC_HD float mf_b(const GatherArgs<cf> &g, int v0, int nv, int kp, int slot, long long off, bool ok) {
+    const int v = kp / 6;
+    if (!ok || v >= nv) return 0.0f;
+    const int j = (kp % 6) >> 1;
+    const cf x = g.xin[off + (size_t)(3 * (v0 + v) + j) * g.B + slot];
+    return (kp & 1) ? x.y : x.x;
+}
+
+// exchange-buffer index of output element e of lane l in tile (rt, st)
+GC_HD int mf_xidx(int lane, int e, int rt, int st) { return (16 * rt + mf_row(lane, e)) * 64 + 16 * st + (lane & 15); }
+
+// the lane's slot: its near-zone outputs from the exchange buffer into acc
+template <int NT>
+GC_HD void mf_combine(const float *xb, int lane, cf (*acc)[3]) {
+#pragma unroll
+    for (int u = 0; u < NT; ++u)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            acc[u][i].x += xb[(2 * (3 * u + i)) * 64 + lane];
+            acc[u][i].y += xb[(2 * (3 * u + i) + 1) * 64 + lane];
+        }
+}
+
 }  // namespace gcore
 }  // namespace scadyn

@@ -13,12 +13,16 @@
 // unwritten), and
 // batch widths 64, 112 (partial second chunk), 56 and 40.
 //
+// The complex64 matrix-core pass (k_gather_mf) runs with the instruction
+// emulated per tile.
+//
 // Exit status 0 when every case agrees to 1e-13 (complex128) / 1e-5 (complex64).
 #include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdio>
 #include <random>
+#include <type_traits>
 #include <vector>
 
 #include "gather_core.hpp"
@@ -57,6 +61,96 @@ static void emulate_pass(const GatherArgs<C> &g, const std::vector<Item> &it, in
     for (int l = 0; l < 64; ++l) write_out<C, NT>(g, it[l], tp, (C(*)[3])acc[l].data());
 }
 
+// The complex64 pass with the near zone on the matrix cores (k_gather_mf):
+// the same operand functions per lane, v_mfma_f32_16x16x4f32 emulated per
+// tile (D[m][n] += sum_k A[m][k] B[k][n], A[m][k] from lane m + 16 k, B[k][n]
+// from lane n + 16 k, output (m, n) to lane n + 16 (m / 4)'s element m % 4 as
+// mf_row states), the kernel's phase order and exchange buffer.
+template <int NT, int GT, int VC>
+static void emulate_pass_mf(const GatherArgs<cf> &g, const std::vector<Item> &it, int tp, std::vector<cd> &smbuf) {
+    cf *sm = (cf *)smbuf.data();
+    cf *smb = (cf *)smbuf.data();
+    float *xb = (float *)smbuf.data();
+    constexpr int RT = mf_rows(NT);
+    std::vector<std::array<std::array<std::array<float, 4>, 4>, RT>> D(64);
+    for (auto &d : D)
+        for (auto &a : d)
+            for (auto &b : a) b.fill(0.0f);
+    std::vector<int64_t> rsu(64);
+    std::vector<std::array<long long, 4>> off(64);
+    for (int l = 0; l < 64; ++l) {
+        rsu[l] = lane_row_start<cf, GT, VC>(g, tp, NT, l);
+        const int s0 = (it[l].b & ~63) + (l & 15);
+        for (int st = 0; st < 4; ++st) off[l][st] = s0 + 16 * st < g.B ? g.in_off[s0 + 16 * st] : 0;
+    }
+    int q = 0;
+    for_near_cubes(g, it[0], [&](int c0, int c1) {
+        for (int v0 = c0; v0 < c1; v0 += VC) {
+            const int nv = std::min(VC, c1 - v0);
+            for (int l = 0; l < 64; ++l) stage_blocks<cf, GT, VC>(g, NT, nv, q, l, rsu[l], smb);
+            const int S = (6 * nv + 3) / 4;
+            for (int s = 0; s < S; ++s) {
+                float ao[64][RT], bo[64][4];
+                for (int l = 0; l < 64; ++l) {
+                    const int kp = 4 * s + (l >> 4), s0 = (it[l].b & ~63) + (l & 15);
+                    for (int st = 0; st < 4; ++st)
+                        bo[l][st] = mf_b(g, v0, nv, kp, s0 + 16 * st, off[l][st], s0 + 16 * st < g.B);
+                    for (int rt = 0; rt < RT; ++rt) ao[l][rt] = mf_a(smb, NT, nv, VC, 16 * rt + (l & 15), kp);
+                }
+                for (int rt = 0; rt < RT; ++rt)
+                    for (int st = 0; st < 4; ++st) {
+                        float tile[16][16];
+                        for (int m = 0; m < 16; ++m)
+                            for (int n = 0; n < 16; ++n) {
+                                float a = 0.0f;
+                                for (int k = 0; k < 4; ++k) a += ao[m + 16 * k][rt] * bo[n + 16 * k][st];
+                                tile[m][n] = a;
+                            }
+                        for (int l = 0; l < 64; ++l)
+                            for (int e = 0; e < 4; ++e) D[l][rt][st][e] += tile[mf_row(l, e)][l & 15];
+                    }
+            }
+            q += nv;
+        }
+    });
+    for (int l = 0; l < 64; ++l)
+        for (int rt = 0; rt < RT; ++rt)
+            for (int st = 0; st < 4; ++st)
+                for (int e = 0; e < 4; ++e) xb[mf_xidx(l, e, rt, st)] = D[l][rt][st][e];
+    std::vector<std::array<std::array<cf, 3>, NT>> acc(64);
+    for (int l = 0; l < 64; ++l) {
+        zero_acc<cf, NT>((cf(*)[3])acc[l].data());
+        mf_combine<NT>(xb, l, (cf(*)[3])acc[l].data());
+    }
+    for (int l = 0; l < 64; ++l) stage_s4(g, tp, NT, l, sm);
+    for (int l = 0; l < 64; ++l) grid_part<cf, NT>(g, it[l], sm, (cf(*)[3])acc[l].data());
+    for (int l = 0; l < 64; ++l) write_out<cf, NT>(g, it[l], tp, (cf(*)[3])acc[l].data());
+}
+
+template <int GT, int VC>
+static void emulate_mf(const GatherArgs<cf> &g) {
+    constexpr int SB = Shape<cf, GT, VC>::BYTES > MF_XBYTES ? Shape<cf, GT, VC>::BYTES : MF_XBYTES;
+    std::vector<cd> sm((SB + 15) / 16);  // exact size, 16-byte aligned
+    const long long nw = (long long)g.n_ne * ((g.B + 63) >> 6);
+    for (long long b = 0; b < nw; ++b) {
+        const long long w = xcd_remap(b, nw);
+        std::vector<Item> it(64);
+        for (int l = 0; l < 64; ++l) it[l] = make_item(g, w, l);
+        for (int tp = it[0].t0; tp < it[0].t1; tp += GT) {
+            switch (std::min(GT, it[0].t1 - tp)) {
+                case 1: emulate_pass_mf<1, GT, VC>(g, it, tp, sm); break;
+                case 2: emulate_pass_mf<2, GT, VC>(g, it, tp, sm); break;
+                case 3: emulate_pass_mf<3, GT, VC>(g, it, tp, sm); break;
+                case 4: emulate_pass_mf<4, GT, VC>(g, it, tp, sm); break;
+                case 5: emulate_pass_mf<5, GT, VC>(g, it, tp, sm); break;
+                case 6: emulate_pass_mf<6, GT, VC>(g, it, tp, sm); break;
+                case 7: emulate_pass_mf<7, GT, VC>(g, it, tp, sm); break;
+                default: emulate_pass_mf<8, GT, VC>(g, it, tp, sm); break;
+            }
+        }
+    }
+}
+
 template <typename C, int GT, int VC>
 static void emulate(const GatherArgs<C> &g) {
     static_assert(GT == 8, "pass switch covers 1..8");
@@ -82,7 +176,7 @@ static void emulate(const GatherArgs<C> &g) {
 }
 
 template <typename C>
-static void run_case(int B, std::mt19937_64 &rng) {
+static void run_case(int B, std::mt19937_64 &rng, bool mf = false) {
     const int ncx = 4, ncy = 3, ncz = 3;
     const int Nx = 2 * ncx + 2, Ny = 2 * ncy + 2, Nz = 2 * ncz + 2;
     const int ncubes = ncx * ncy * ncz;
@@ -135,7 +229,14 @@ static void run_case(int B, std::mt19937_64 &rng) {
     GatherArgs<C> g = {grid.data(), xin.data(), in_off.data(), B, ne.data(), (int)ne.size(), cube_start.data(),
                        S4.data(), Nx, Ny, ncx, ncy, ncz, row_start.data(), ent_blk.data(), blocks.data(), k2,
                        active.data(), y.data()};
-    emulate<C, GATHER_GT, GATHER_VC>(g);
+    if constexpr (std::is_same<C, cf>::value) {
+        if (mf)
+            emulate_mf<GATHER_GT, GATHER_VC>(g);
+        else
+            emulate<C, GATHER_GT, GATHER_VC>(g);
+    } else {
+        emulate<C, GATHER_GT, GATHER_VC>(g);
+    }
     // direct evaluation in double
     double num = 0.0, den = 0.0;
     for (int t = 0; t < N; ++t) {
@@ -179,7 +280,7 @@ static void run_case(int B, std::mt19937_64 &rng) {
     const bool ok = err < tol;
     if (!ok) ++failures;
     std::printf("%s B=%3d N=%4d cubes %d (non-empty %zu) entries %zu  rel.err %.2e %s\n",
-                sizeof(C) == 16 ? "c128" : "c64 ", B, N, ncubes, ne.size(), E, err, ok ? "ok" : "FAIL");
+                sizeof(C) == 16 ? "c128" : (mf ? "c64m" : "c64 "), B, N, ncubes, ne.size(), E, err, ok ? "ok" : "FAIL");
 }
 
 // xcd_remap is a bijection of [0, G) that gives XCD b % 8 one contiguous range
@@ -216,6 +317,7 @@ int main() {
     for (int B : {64, 112, 56, 40}) {
         run_case<cd>(B, rng);
         run_case<cf>(B, rng);
+        run_case<cf>(B, rng, true);  // near zone on the (emulated) matrix cores
     }
     std::printf("%d failures\n", failures);
     return failures ? 1 : 0;

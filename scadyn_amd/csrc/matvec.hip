@@ -6,6 +6,9 @@
 // (the three "Y" convolutions) and channel 3 carries Sx^T x_x + Sy^T x_y +
 // Sz^T x_z; by linearity this one convolution replaces the reference's three
 // divergence convolutions (:122-144), whose gathered sum it equals.
+#include <cstdlib>
+#include <type_traits>
+
 #include "matvec.hpp"
 #include "gather_core.hpp"
 
@@ -414,6 +417,106 @@ __global__ __launch_bounds__(64, 3) void k_gather_lds(gcore::GatherArgs<C> g) {
     }
 }
 
+// The complex64 gather with the near-zone product on the matrix cores
+// (v_mfma_f32_16x16x4f32, operand algebra in gather_core.hpp): the grid part
+// as k_gather_lds, then per near chunk the blocks staged in LDS as the A
+// operand and the column tets' x as the B operand, the output tiles exchanged
+// through LDS to the lanes that own the slots.  FP32 matrix instructions
+// issue at twice the FP32 vector rate, and the stored complex64 blocks need
+// no conversion.
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <int NT, int GT, int VC>
+__device__ __forceinline__ void gather_pass_mf(const gcore::GatherArgs<cf> &g, const gcore::Item &it, int tp,
+                                               int lane, unsigned char *smraw) {
+    using namespace gcore;
+    cf *sm = (cf *)smraw;
+    cf *smb = (cf *)smraw;
+    float *xb = (float *)smraw;
+    // near part first (its output tiles in registers), then the tiles through
+    // LDS into the lane's accumulators, then the grid part: the two phases'
+    // accumulators are never live together
+    constexpr int RT = mf_rows(NT);
+    v4f D[RT][4];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int st = 0; st < 4; ++st) D[rt][st] = (v4f){0.0f, 0.0f, 0.0f, 0.0f};
+    const int lo = lane & 15, kq = lane >> 4;
+    const int s0 = (it.b & ~63) + lo;  // slot of tile 0 for this lane's B operand; tile st adds 16 st
+    long long off[4];
+#pragma unroll
+    for (int st = 0; st < 4; ++st) off[st] = s0 + 16 * st < g.B ? g.in_off[s0 + 16 * st] : 0;
+    const int64_t rsu = lane_row_start<cf, GT, VC>(g, tp, NT, lane);
+    int q = 0;
+    for_near_cubes(g, it, [&](int c0, int c1) {
+        for (int v0 = c0; v0 < c1; v0 += VC) {
+            const int nv = min(VC, c1 - v0);
+            __syncthreads();  // the previous chunk's (or pass's) reads of the buffer are done
+            stage_blocks<cf, GT, VC>(g, NT, nv, q, lane, rsu, smb);
+            __syncthreads();
+            const int S = (6 * nv + 3) / 4;
+            for (int s = 0; s < S; ++s) {
+                const int kp = 4 * s + kq;
+                float bo[4], ao[RT];
+#pragma unroll
+                for (int st = 0; st < 4; ++st)
+                    bo[st] = mf_b(g, v0, nv, kp, s0 + 16 * st, off[st], s0 + 16 * st < g.B);
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt) ao[rt] = mf_a(smb, NT, nv, VC, 16 * rt + lo, kp);
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+                    for (int st = 0; st < 4; ++st)
+                        D[rt][st] = __builtin_amdgcn_mfma_f32_16x16x4f32(ao[rt], bo[st], D[rt][st], 0, 0, 0);
+            }
+            q += nv;
+        }
+    });
+    __syncthreads();  // the last chunk's block reads are done
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int st = 0; st < 4; ++st)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) xb[mf_xidx(lane, e, rt, st)] = D[rt][st][e];
+    __syncthreads();
+    cf acc[NT][3];
+    zero_acc<cf, NT>(acc);
+    mf_combine<NT>(xb, lane, acc);
+    __syncthreads();  // the exchange is read before the S* records overwrite it
+    stage_s4(g, tp, NT, lane, sm);
+    __syncthreads();
+    grid_part<cf, NT>(g, it, sm, acc);
+    write_out<cf, NT>(g, it, tp, acc);
+}
+
+template <int GT, int VC>
+__global__ __launch_bounds__(64, 3) void k_gather_mf(gcore::GatherArgs<cf> g) {
+    using namespace gcore;
+    static_assert(GT == 8, "the pass switch below covers 1..8 row tets");
+    constexpr int SB = Shape<cf, GT, VC>::BYTES > MF_XBYTES ? Shape<cf, GT, VC>::BYTES : MF_XBYTES;
+    __shared__ __align__(16) unsigned char sm[SB];
+    const int lane = threadIdx.x;
+    const long long nw = (long long)g.n_ne * ((g.B + 63) >> 6);
+    for (long long b = blockIdx.x; b < nw; b += gridDim.x) {
+        const long long w = nw == gridDim.x ? xcd_remap(b, nw) : b;
+        const Item it = make_item(g, w, lane);
+        for (int tp = it.t0; tp < it.t1; tp += GT) {
+            switch (min(GT, it.t1 - tp)) {  // wave-uniform
+                case 1: gather_pass_mf<1, GT, VC>(g, it, tp, lane, sm); break;
+                case 2: gather_pass_mf<2, GT, VC>(g, it, tp, lane, sm); break;
+                case 3: gather_pass_mf<3, GT, VC>(g, it, tp, lane, sm); break;
+                case 4: gather_pass_mf<4, GT, VC>(g, it, tp, lane, sm); break;
+                case 5: gather_pass_mf<5, GT, VC>(g, it, tp, lane, sm); break;
+                case 6: gather_pass_mf<6, GT, VC>(g, it, tp, lane, sm); break;
+                case 7: gather_pass_mf<7, GT, VC>(g, it, tp, lane, sm); break;
+                default: gather_pass_mf<8, GT, VC>(g, it, tp, lane, sm); break;
+            }
+        }
+    }
+}
+
 // slot-chunk (wave64-per-item) kernels when the last 64-slot chunk is at least
 // half full: even then they beat the slot-per-lane kernels, which share no
 // operand loads between the tets of a cube or the nodes of a block
@@ -439,6 +542,16 @@ void launch_scatter(const MatvecArgs<C> &a, hipStream_t s) {
     HIP_CHECK(hipGetLastError());
 }
 
+// the complex64 gather's near zone on the matrix cores (SCADYN_GATHER_MFMA=0
+// selects the vector-FMA kernel, for comparisons)
+static bool gather_mfma() {
+    static const bool on = [] {
+        const char *e = std::getenv("SCADYN_GATHER_MFMA");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 template <typename C>
 void launch_gather_near(const MatvecArgs<C> &a, hipStream_t s) {
     if (wave_slots(a.B) && a.ne_cubes) {
@@ -447,6 +560,14 @@ void launch_gather_near(const MatvecArgs<C> &a, hipStream_t s) {
                                         a.n_ne,     a.cube_start, a.S4,      a.Nx,     a.Ny,
                                         a.ncx,      a.ncy,       a.ncz,       a.row_start, a.ent_blk,
                                         a.blocks,   a.k2,        a.slot_active, a.y};
+        if constexpr (std::is_same<C, cf>::value) {
+            if (gather_mfma()) {
+                hipLaunchKernelGGL((k_gather_mf<gcore::GATHER_GT, gcore::GATHER_VC>), dim3(grid_for(nw, 1)), dim3(64),
+                                   0, s, g);
+                HIP_CHECK(hipGetLastError());
+                return;
+            }
+        }
         hipLaunchKernelGGL((k_gather_lds<C, gcore::GATHER_GT, gcore::GATHER_VC>), dim3(grid_for(nw, 1)), dim3(64), 0, s,
                            g);
         HIP_CHECK(hipGetLastError());
