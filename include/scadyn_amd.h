@@ -91,7 +91,10 @@ typedef struct scadyn_opts {
                             rows evaluates (in-pass Gram correction, same recurrence):
                             0 automatic (4), 1, 2, 4, 6 or 8; a pass moves about
                             2 + 2/mgs_block vectors of HBM traffic per coefficient      */
-    int32_t reserved[5];
+    int32_t gather_kernel; /* complex64 gather + near zone: 0 vector FMA (default), 1 the near-zone
+                              product on the matrix cores (v_mfma_f32_16x16x4f32; measured about 2x
+                              slower, its operand set-up costs as much as the FMAs it replaces) */
+    int32_t reserved[4];
 } scadyn_opts;
 
 /* Defaults of the reference (src/common.f90:78-89). */

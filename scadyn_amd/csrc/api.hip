@@ -609,6 +609,7 @@ static void run_matvec(scadyn_operator *op, MatvecSpace &w, const C *xin, const 
     a.n_ne = op->n_ne_cubes;
     a.k2 = op->k * op->k;
     a.slot_active = active;
+    a.gather_kernel = op->opts.gather_kernel;
     a.grid_in = g0;
     a.grid_out = g0;
     a.y = y;
@@ -1583,6 +1584,8 @@ int scadyn_op_create(const scadyn_mesh *mesh, const scadyn_opts *opts, double k,
         fail(SCADYN_ERR_UNSUPPORTED, "block MGS (ortho = 2) is disabled on the device: its first run faulted and "
                                      "the cause is not yet found");
     if (o.fft_kernel != 0 && o.fft_kernel != 1) fail(SCADYN_ERR_ARG, "fft_kernel must be 0 (automatic) or 1 (Stockham)");
+    if (o.gather_kernel != 0 && o.gather_kernel != 1)
+        fail(SCADYN_ERR_ARG, "gather_kernel must be 0 (vector FMA) or 1 (matrix cores)");
     if (o.mgs_block != 0 && o.mgs_block != 1 && o.mgs_block != 2 && o.mgs_block != 4 && o.mgs_block != 6 &&
         o.mgs_block != 8)
         fail(SCADYN_ERR_ARG, "mgs_block must be 0 (automatic), 1, 2, 4, 6 or 8");

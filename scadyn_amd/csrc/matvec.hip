@@ -6,7 +6,6 @@
 // (the three "Y" convolutions) and channel 3 carries Sx^T x_x + Sy^T x_y +
 // Sz^T x_z; by linearity this one convolution replaces the reference's three
 // divergence convolutions (:122-144), whose gathered sum it equals.
-#include <cstdlib>
 #include <type_traits>
 
 #include "matvec.hpp"
@@ -421,9 +420,12 @@ __global__ __launch_bounds__(64, 3) void k_gather_lds(gcore::GatherArgs<C> g) {
 // (v_mfma_f32_16x16x4f32, operand algebra in gather_core.hpp): the grid part
 // as k_gather_lds, then per near chunk the blocks staged in LDS as the A
 // operand and the column tets' x as the B operand, the output tiles exchanged
-// through LDS to the lanes that own the slots.  FP32 matrix instructions
-// issue at twice the FP32 vector rate, and the stored complex64 blocks need
-// no conversion.
+// through LDS to the lanes that own the slots (opts.gather_kernel = 1).  FP32
+// matrix instructions have twice the FP32 vector rate and the stored
+// complex64 blocks need no conversion, but each k-step's operand set-up
+// (index decode, LDS read, x load) costs about as many vector instructions as
+// the FMAs it replaces: measured 23.0 vs 10.8 ms per launch at 824k tets,
+// 59 slots, so the vector-FMA kernel stays the default.
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 template <int NT, int GT, int VC>
@@ -456,14 +458,27 @@ __device__ __forceinline__ void gather_pass_mf(const gcore::GatherArgs<cf> &g, c
             stage_blocks<cf, GT, VC>(g, NT, nv, q, lane, rsu, smb);
             __syncthreads();
             const int S = (6 * nv + 3) / 4;
+            // operands fetched one k-step ahead, so that the loads of step s + 1
+            // are in flight while the matrix instructions of step s run
+            float bn[4], an[RT];
+#pragma unroll
+            for (int st = 0; st < 4; ++st) bn[st] = mf_b(g, v0, nv, kq, s0 + 16 * st, off[st], s0 + 16 * st < g.B);
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) an[rt] = mf_a(smb, NT, nv, VC, 16 * rt + lo, kq);
             for (int s = 0; s < S; ++s) {
-                const int kp = 4 * s + kq;
                 float bo[4], ao[RT];
 #pragma unroll
-                for (int st = 0; st < 4; ++st)
-                    bo[st] = mf_b(g, v0, nv, kp, s0 + 16 * st, off[st], s0 + 16 * st < g.B);
+                for (int st = 0; st < 4; ++st) bo[st] = bn[st];
 #pragma unroll
-                for (int rt = 0; rt < RT; ++rt) ao[rt] = mf_a(smb, NT, nv, VC, 16 * rt + lo, kp);
+                for (int rt = 0; rt < RT; ++rt) ao[rt] = an[rt];
+                if (s + 1 < S) {
+                    const int kp = 4 * (s + 1) + kq;
+#pragma unroll
+                    for (int st = 0; st < 4; ++st)
+                        bn[st] = mf_b(g, v0, nv, kp, s0 + 16 * st, off[st], s0 + 16 * st < g.B);
+#pragma unroll
+                    for (int rt = 0; rt < RT; ++rt) an[rt] = mf_a(smb, NT, nv, VC, 16 * rt + lo, kp);
+                }
 #pragma unroll
                 for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
@@ -542,16 +557,6 @@ void launch_scatter(const MatvecArgs<C> &a, hipStream_t s) {
     HIP_CHECK(hipGetLastError());
 }
 
-// the complex64 gather's near zone on the matrix cores (SCADYN_GATHER_MFMA=0
-// selects the vector-FMA kernel, for comparisons)
-static bool gather_mfma() {
-    static const bool on = [] {
-        const char *e = std::getenv("SCADYN_GATHER_MFMA");
-        return !(e && e[0] == '0');
-    }();
-    return on;
-}
-
 template <typename C>
 void launch_gather_near(const MatvecArgs<C> &a, hipStream_t s) {
     if (wave_slots(a.B) && a.ne_cubes) {
@@ -561,7 +566,7 @@ void launch_gather_near(const MatvecArgs<C> &a, hipStream_t s) {
                                         a.ncx,      a.ncy,       a.ncz,       a.row_start, a.ent_blk,
                                         a.blocks,   a.k2,        a.slot_active, a.y};
         if constexpr (std::is_same<C, cf>::value) {
-            if (gather_mfma()) {
+            if (a.gather_kernel == 1) {
                 hipLaunchKernelGGL((k_gather_mf<gcore::GATHER_GT, gcore::GATHER_VC>), dim3(grid_for(nw, 1)), dim3(64),
                                    0, s, g);
                 HIP_CHECK(hipGetLastError());

@@ -27,6 +27,7 @@ struct MatvecArgs {
     int n_ne = 0;
     double k2 = 0;
     const int *slot_active = nullptr;  // optional per-slot mask
+    int gather_kernel = 0;             // complex64: 1 = near zone on the matrix cores
     C *grid_in = nullptr;              // scatter output [Nz][Ny][Nx][4][B]
     const C *grid_out = nullptr;       // convolved grid, same layout
     cd *y = nullptr;                   // output [3N][B]

@@ -23,8 +23,8 @@ module scadyn_amd_iface
       real(c_double) :: grid_size
       integer(c_int32_t) :: order, precision, batch, strict, device, fft_pad
       real(c_double) :: refine_tol
-      integer(c_int32_t) :: ortho, fft_kernel, mgs_block
-      integer(c_int32_t) :: reserved(5)
+      integer(c_int32_t) :: ortho, fft_kernel, mgs_block, gather_kernel
+      integer(c_int32_t) :: reserved(4)
    end type
 
    interface

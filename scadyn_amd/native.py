@@ -54,7 +54,7 @@ class COpts(ctypes.Structure):
                 ('order', ctypes.c_int32), ('precision', ctypes.c_int32), ('batch', ctypes.c_int32),
                 ('strict', ctypes.c_int32), ('device', ctypes.c_int32), ('fft_pad', ctypes.c_int32),
                 ('refine_tol', ctypes.c_double), ('ortho', ctypes.c_int32), ('fft_kernel', ctypes.c_int32),
-                ('mgs_block', ctypes.c_int32), ('reserved', ctypes.c_int32 * 5)]
+                ('mgs_block', ctypes.c_int32), ('gather_kernel', ctypes.c_int32), ('reserved', ctypes.c_int32 * 4)]
 
 
 _lib = None

@@ -90,3 +90,23 @@ def test_matvec_dense_cubes_wide_vs_narrow(wide):
     assert ea < 1e-3 and ea < 4 * eb
     a.close()
     b.close()
+
+
+def test_matvec_gather_on_matrix_cores():
+    """The complex64 gather with the near-zone product on the matrix cores
+    (gather_kernel = 1) against the vector-FMA kernel, on a mesh whose cubes
+    hold up to 17 tets (several row passes per cube), partial slot chunks."""
+    import numpy as np
+    from helpers import refined_sphere_case, rel
+    from scadyn_amd import native
+    c, e, p, k = refined_sphere_case()
+    rng = np.random.default_rng(2)
+    for B in (64, 40):
+        x = rng.standard_normal((B, 3 * e.shape[0])) + 1j * rng.standard_normal((B, 3 * e.shape[0]))
+        ys = []
+        for gk in (0, 1):
+            op = native.Operator(c, e, p, k, batch=B, gather_kernel=gk)
+            ys.append(op.matvec(x, precision=native.PREC_MIXED))
+            op.close()
+        print(f'[gpu-batch64] matrix-core gather, B={B}: {rel(ys[1], ys[0]):.2e}', flush=True)
+        assert rel(ys[1], ys[0]) < 2e-6
