@@ -126,7 +126,9 @@ def run_leg(name, op, nmax, order, K, warmup, steps, prec_key, spread):
          'ms_per_step': 1000.0 * elapsed / max(steps, 1), 'steps': steps, 'warmup': warmup,
          'batch': info['batch'], 'mean_iterations': float(it[fin].mean()) if fin.any() else None,
          'max_iterations': int(it[fin].max()) if fin.any() else None,
-         'columns_at_iteration_cap': int(np.sum(it[fin] >= cap)), 'kernels': kernel_table(tim),
+         'columns_at_iteration_cap': int(np.sum(it[fin] >= cap)),
+         # the reference's re-orthogonalisation test (its extra pass is not applied)
+         'reorth_test_fired': info['reorth_fired'], 'kernels': kernel_table(tim),
          'fft_tiles': info['fft_tiles' if prec_key == 'fp64' else 'fft_tiles_f32']}
     say(f"{name}: {r['columns_per_s']:.3f} columns/s ({ncols} columns in {elapsed:.1f} s), batch {r['batch']}, "
         f"mean iterations {r['mean_iterations']}")

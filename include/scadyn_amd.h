@@ -133,7 +133,9 @@ void scadyn_op_destroy(scadyn_operator *op);
  * solve, 1 if the near-zone rows are in cube order (per-cube gather), then the
  * tile widths of the five line-FFT passes of the current batch in FP64 and in
  * FP32 (8 bits each: x, y, z, inverse y, inverse x; tuned for large grids);
- * delta, box_delta and k in dinfo[0..2]. */
+ * delta, box_delta and k in dinfo[0..2]; dinfo[3] = times the reference's
+ * re-orthogonalisation test (src/gmres_module.f90:251) fired, summed over the
+ * systems finished on this operator (the extra pass itself is not applied). */
 int scadyn_op_info(const scadyn_operator *op, int64_t info[16], double dinfo[4]);
 
 #define SCADYN_EXPORT_S       1  /* S,Sx,Sy,Sz: complex [4][N][27], reference tet order  */

@@ -22,18 +22,31 @@ Numerical quirks of the reference kept on purpose (they change results):
   * near-zone blocks are rounded to complex64 (src/precorrection.f90:251);
   * the pseudo-inverse drops singular values <= 1e-6 (src/possu.f90:413).
 
-Pinning status of each function (the tests that pin them are in tests/):
-  quadrature tables       pinned (exact monomial integration)
-  analytical_integration  pinned (brute-force quadrature of 1/R and R)
-  int_V_G / int_S_G       pinned (adaptive brute force, near and singular)
-  build_G + FFT conv.     pinned (direct O(n^2) Green's sum on a small grid)
-  projection S*           pinned (fit residual + grid-translation invariance)
-  near zone + matvec      pinned (near-pair identity: operator == exact block;
-                          A^T == A on a dense assembly)
-  GMRES                   pinned (dense numpy.linalg.solve)
-  VSWF                    pinned (closed-form n=1 waves; scipy special functions)
-  T-matrix                pinned (Mie series for a sphere, discretisation-level;
-                          reciprocity symmetry of T)
+Pinning status of each function (tests/ holds the checks):
+  quadrature tables       pinned: exact monomial integration
+                          (test_oracle_quadrature.py) and bit-equal to the
+                          compiled reference module (test_oracle_vs_ref.py)
+  analytical_integration  pinned: brute-force quadrature of 1/R and R
+                          (test_oracle_singular.py); equal to the compiled
+                          reference to 1e-12 (test_oracle_vs_ref.py)
+  int_V_G / int_S_G       pinned: adaptive brute force, near and singular;
+                          equal to the compiled reference, with the V_V and
+                          dV_dV pair integrals
+  geometry                pinned: bit-equal to the compiled reference's
+                          build_grid / build_box / tetras_in_cubes
+  build_G + FFT conv.     pinned: direct O(n^2) Green's sum on a small grid
+                          (test_oracle_operator.py)
+  projection S*           pinned: fit residual and grid-translation invariance
+  near zone + matvec      pinned: near-pair identity (operator == exact block)
+                          and the symmetry defect (self blocks only)
+  GMRES                   pinned: dense numpy.linalg.solve; restart semantics
+  VSWF                    pinned: curl identities, |M|^2 normalisation, phase
+                          convention, closed-form M_10 (test_oracle_vswf.py)
+  T-matrix                pinned: the exact Galerkin operator's T converges to
+                          Mie at about O(h^2) on faceted balls, planted
+                          operator mistakes fail it, the pFFT operator and T
+                          stay close to the exact ones, reciprocity on an
+                          asymmetric body (test_oracle_tmatrix_pin.py)
 """
 import numpy as np
 

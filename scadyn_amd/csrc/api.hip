@@ -192,6 +192,9 @@ struct scadyn_operator {
     std::unique_ptr<Workspace> ws;
     Profiler prof;
     int32_t last_outer_rounds = 0;
+    // times the reference's re-orthogonalisation test (src/gmres_module.f90:251)
+    // fired, summed over the systems finished on this operator
+    long long reorth_fired = 0;
     ~scadyn_operator() {
         ws.reset();
         mem.clear();
@@ -1042,6 +1045,7 @@ struct ColumnSolver {
             if (slot_col[b] >= 0 && hs.p[b].phase == PH_DONE) {
                 h_ret.p[b] = slot_col[b];
                 if (iters) iters[slot_col[b]] += hs.p[b].ite;
+                op->reorth_fired += hs.p[b].reorth;
                 max_cycles = std::max(max_cycles, hs.p[b].cycle + 1);
                 if (hs.p[b].conv == 0) nfail++;
                 if (hs.p[b].conv < 0) singular++;
@@ -1631,7 +1635,7 @@ int scadyn_op_info(const scadyn_operator *op, int64_t info[16], double dinfo[4])
         dinfo[0] = op->g.delta;
         dinfo[1] = op->g.box_delta;
         dinfo[2] = op->k;
-        dinfo[3] = 0;
+        dinfo[3] = (double)op->reorth_fired;
     }
     ABI_CATCH
 }

@@ -191,6 +191,7 @@ class Operator:
         for k in ('fft_tiles', 'fft_tiles_f32'):  # x, y, z, inverse y, inverse x
             out[k] = [(out[k] >> (8 * q)) & 0xff for q in range(5)]
         out['delta'], out['box_delta'] = float(d[0]), float(d[1])
+        out['reorth_fired'] = int(d[3])
         return out
 
     def export(self, what):
