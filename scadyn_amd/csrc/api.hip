@@ -692,7 +692,11 @@ static void tune_fft_tiles(scadyn_operator *op, MatvecSpace &ms) {
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
-    const size_t budget = 64 * 1024;  // same LDS budget as the planner
+    // candidates may use the device's whole LDS per workgroup (160 KiB on MI355X;
+    // the planner's defaults stay within 64 KiB)
+    int lds_dev = 0;
+    HIP_CHECK(hipDeviceGetAttribute(&lds_dev, hipDeviceAttributeMaxSharedMemoryPerBlock, op->device));
+    const size_t budget = std::max<size_t>(64 * 1024, (size_t)lds_dev);
     const int nprec = op->opts.precision == SCADYN_PREC_MIXED ? 2 : 1;
     for (int pr = 0; pr < nprec; ++pr) {
         if (pr == 1 && !op->d_fgf) break;  // no complex64 operator data yet

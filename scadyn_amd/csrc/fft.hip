@@ -267,10 +267,11 @@ size_t line_pass_lds(const LinePass &p, size_t elem) {
 }
 
 // Register-kernel tile width ti: the largest power of two <= want that fits the
-// inner extent, the LDS budget and 512 threads.
+// inner extent, the LDS budget and the kernel's thread bound.
 static int reg_tile(int L, int rmax, long long inner, size_t lds_budget, size_t elem, int want) {
+    const int tmax = reg_max_threads(elem);
     int ti = want;
-    while (ti > 1 && (ti > inner || elem * ((size_t)L + (size_t)L * ti) > lds_budget || ti * rmax > 512)) ti /= 2;
+    while (ti > 1 && (ti > inner || elem * ((size_t)L + (size_t)L * ti) > lds_budget || ti * rmax > tmax)) ti /= 2;
     return ti;
 }
 

@@ -24,6 +24,11 @@ struct LinePass {
     int threads = 256;    // workgroup size
 };
 
+// Thread bound of the register line-FFT kernel: complex64 instances take at
+// most 92 VGPRs, so they launch up to 1024 threads (4 waves per SIMD);
+// complex128 instances (up to 174 VGPRs) stay at 512.
+inline int reg_max_threads(size_t elem) { return elem <= 8 ? 1024 : 512; }
+
 // Fg lookup for the fused z-pass: inner index i of the [Ly][Lx][nb] layout
 // -> frequency (kx, ky); kz is the line index.  Fg holds the even octant.
 struct ConvInfo {

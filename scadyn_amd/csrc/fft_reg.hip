@@ -6,7 +6,7 @@ namespace scadyn {
 namespace regfft {
 
 template <int MODE, typename C, int RMAX>
-__global__ __launch_bounds__(512) void k_line_reg(const C *__restrict__ in, C *__restrict__ out, LinePass p,
+__global__ __launch_bounds__(sizeof(C) <= 8 ? 1024 : 512) void k_line_reg(const C *__restrict__ in, C *__restrict__ out, LinePass p,
                                                   const C *__restrict__ twg, int sign, const C *__restrict__ fg,
                                                   ConvInfo ci) {
     extern __shared__ char smraw[];
@@ -48,8 +48,22 @@ void launch_line_reg(const LinePass &p, const C *in, C *out, const C *tw, int si
     if (blocks <= 0) return;
     if (blocks > 0x7fffffffLL) fail(SCADYN_ERR_UNSUPPORTED, "FFT grid too large");
     const size_t lds = line_pass_lds(p, sizeof(C));
+    if (p.threads > reg_max_threads(sizeof(C))) fail(SCADYN_ERR_UNSUPPORTED, "FFT tile exceeds the thread bound");
     const dim3 grid((unsigned)blocks), block(p.threads);
     const int rm = reg_rmax(p);
+    if (lds > 65536) {  // tiles beyond 64 KiB of LDS (MI355X allows 160 KiB per workgroup)
+#define ATTR(M, R)                                                                                        \
+    HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_line_reg<M, C, R>),                  \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        if (mode == 0) {
+            if (rm == 12) { ATTR(0, 12) } else if (rm == 14) { ATTR(0, 14) } else if (rm == 16) { ATTR(0, 16) }
+            else if (rm == 18) { ATTR(0, 18) } else { ATTR(0, 20) }
+        } else {
+            if (rm == 12) { ATTR(1, 12) } else if (rm == 14) { ATTR(1, 14) } else if (rm == 16) { ATTR(1, 16) }
+            else if (rm == 18) { ATTR(1, 18) } else { ATTR(1, 20) }
+        }
+#undef ATTR
+    }
     if (mode == 0) {
         if (rm == 12)
             hipLaunchKernelGGL((k_line_reg<0, C, 12>), grid, block, lds, s, in, out, p, tw, sign, fg, ci);

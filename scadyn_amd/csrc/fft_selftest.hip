@@ -101,11 +101,11 @@ static void check(const char *what, int L, const LinePass &p, const std::vector<
 }
 
 template <typename C>
-static void line_case(int L, int sign, std::mt19937_64 &rng) {
+static void line_case(int L, int sign, std::mt19937_64 &rng, long long inner = 19, size_t lds = 64 * 1024) {
     const int n_in = sign < 0 ? L / 2 + 1 : L, n_out = sign < 0 ? L : L / 2 + 1;
-    const long long inner = 19, outer = 2;
+    const long long outer = 2;
     LinePass p;
-    make_line_pass(p, L, n_in, n_out, inner, outer, 64 * 1024, sizeof(C), true);
+    make_line_pass(p, L, n_in, n_out, inner, outer, lds, sizeof(C), true);
     if (!p.reg) {
         std::printf("L=%d not on the register path\n", L);
         ++failures;
@@ -130,12 +130,12 @@ static void line_case(int L, int sign, std::mt19937_64 &rng) {
 // fused z-pass: lines along z of a [Lz][Ly][Lx][nb] grid, zero-padded to n_in = Nz,
 // forward, x Fg(min(kx, Lx-kx), min(ky, Ly-ky), min(kz, Lz-kz)), inverse, cropped to Nz
 template <typename C>
-static void conv_case(int Lz, std::mt19937_64 &rng) {
+static void conv_case(int Lz, std::mt19937_64 &rng, size_t lds = 64 * 1024) {
     const int Nz = Lz / 2, Lx = 4, Ly = 3, nb = 3;
     const int Hx = Lx / 2 + 1, Hy = Ly / 2 + 1, Hz = Lz / 2 + 1;
     const long long inner = (long long)Lx * Ly * nb;
     LinePass p;
-    make_line_pass(p, Lz, Nz, Nz, inner, 1, 64 * 1024, sizeof(C), true);
+    make_line_pass(p, Lz, Nz, Nz, inner, 1, lds, sizeof(C), true);
     ConvInfo ci;
     ci.Lx = Lx;
     ci.Ly = Ly;
@@ -181,6 +181,12 @@ int main() {
         line_case<cf>(L, -1, rng);
         conv_case<cd>(L, rng);
         conv_case<cf>(L, rng);
+        // the widest tiles (32 complex64 columns, up to 1024 threads; 16 complex128
+        // columns) within the 160 KiB of LDS a workgroup may hold, partial last tile
+        line_case<cf>(L, -1, rng, 37, 160 * 1024);
+        line_case<cd>(L, +1, rng, 21, 160 * 1024);
+        conv_case<cf>(L, rng, 160 * 1024);
+        conv_case<cd>(L, rng, 160 * 1024);
     }
     std::printf("%zu lengths, %d failures\n", lengths.size(), failures);
     return failures ? 1 : 0;
