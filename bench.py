@@ -157,7 +157,7 @@ def headline(args):
     ncol = 2 * ((nmax + 1) ** 2 - 1)
     order = np.random.default_rng(0).permutation(ncol)
     say(f'workload {args.workload}: {e.shape[0]} tets, nmax {nmax}, {ncol} columns ({time.time() - t:.1f} s mesh)')
-    common = dict(maxit=args.maxit, restart=args.restart, batch=args.batch)
+    common = dict(maxit=args.maxit, restart=args.restart, batch=args.batch, mgs_block=args.mgs_block)
     legs, Ts = {}, {}
     cpu = None
     for name in ('mixed', 'fp64'):
@@ -329,6 +329,8 @@ def main():
     ap.add_argument('--inner-tol', type=float, default=1e-4,
                     help='mixed: residual reduction that ends a restart cycle early')
     ap.add_argument('--maxit', type=int, default=50, help='Krylov dimension per cycle (reference default 50)')
+    ap.add_argument('--mgs-block', type=int, default=0,
+                    help='MGS coefficients per pass over the rows (0 automatic; 1, 2, 4, 6 or 8)')
     ap.add_argument('--restart', type=int, default=60, help='restart cycles (enough for every column to converge)')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--cpu-budget', type=float, default=25.0, help='seconds of CPU-baseline work')
