@@ -13,7 +13,9 @@ __global__ __launch_bounds__(sizeof(C) <= 8 ? 1024 : 512) void k_line_reg(const 
     C *tw = reinterpret_cast<C *>(smraw);
     C *lds = tw + p.L;
     for (int m = threadIdx.x; m < p.L; m += blockDim.x) tw[m] = twg[m];
-    const Ctx x = make_ctx(p, blockIdx.x, threadIdx.x);
+    // neighbouring tiles of a line row share cache lines (rows need not be
+    // 128-byte aligned): give them to one XCD
+    const Ctx x = make_ctx(p, xcd_remap(blockIdx.x, gridDim.x), threadIdx.x);
     C v[RMAX];
     line_reg_phase<MODE, C, RMAX>(0, in, out, lds, tw, fg, ci, p, x, sign, v);
     __syncthreads();

@@ -43,7 +43,8 @@ static void emulate(const LinePass &p, const C *in, C *out, const C *twg, int si
     std::vector<std::array<C, 20>> regs(p.threads);
     const int nph = MODE == 0 ? 2 : 4;
     const int rm = reg_rmax(p);  // the instance the launcher picks
-    for (long long b = 0; b < blocks; ++b)
+    for (long long bi = 0; bi < blocks; ++bi) {
+        const long long b = xcd_remap(bi, blocks);  // the kernel's workgroup -> tile order
         for (int ph = 0; ph < nph; ++ph)
             for (int t = 0; t < p.threads; ++t) {
                 if (rm == 12)
@@ -62,6 +63,7 @@ static void emulate(const LinePass &p, const C *in, C *out, const C *twg, int si
                     line_reg_phase<MODE, C, 20>(ph, in, out, lds.data(), tw.data(), fg, ci, p, make_ctx(p, b, t),
                                                 sign, regs[t].data());
             }
+    }
 }
 
 template <typename C>

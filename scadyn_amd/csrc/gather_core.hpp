@@ -68,15 +68,6 @@ struct Shape {
     static constexpr int BYTES = S4_BYTES > BLK_BYTES ? S4_BYTES : BLK_BYTES;
 };
 
-// Workgroup b of a G-workgroup launch -> item index, a bijection of [0, G).
-// Workgroups are dealt to the 8 XCDs round-robin (b % 8); XCD x is given the
-// x-th contiguous range of items, the first G % 8 ranges one longer.
-GC_HD long long xcd_remap(long long b, long long G) {
-    const long long per = (G + 7) / 8, tall = G % 8 == 0 ? 8 : G % 8;
-    const long long x = b % 8, j = b / 8;
-    return x < tall ? x * per + j : tall * per + (x - tall) * (per - 1) + j;
-}
-
 // one wave's item: cube and slot
 struct Item {
     int c, cx, cy, cz, t0, t1;

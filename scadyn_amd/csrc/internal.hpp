@@ -32,6 +32,17 @@ void set_error(int code, const std::string &msg);
 typedef double2 cd;
 typedef float2 cf;
 
+// Workgroup b of a G-workgroup launch -> work item, a bijection of [0, G).
+// Workgroups are dealt to the 8 XCDs round-robin (b % 8), each XCD with its own
+// L2; XCD x is given the x-th contiguous range of items, the first G % 8 ranges
+// one longer, so that neighbouring items (which share cache lines) run on the
+// same XCD.
+__host__ __device__ inline long long xcd_remap(long long b, long long G) {
+    const long long per = (G + 7) / 8, tall = G % 8 == 0 ? 8 : G % 8;
+    const long long x = b % 8, j = b / 8;
+    return x < tall ? x * per + j : tall * per + (x - tall) * (per - 1) + j;
+}
+
 __host__ __device__ inline cd cmk(double r, double i) { return make_double2(r, i); }
 __host__ __device__ inline cd cadd(cd a, cd b) { return make_double2(a.x + b.x, a.y + b.y); }
 __host__ __device__ inline cd csub(cd a, cd b) { return make_double2(a.x - b.x, a.y - b.y); }
