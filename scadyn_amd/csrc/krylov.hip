@@ -161,12 +161,9 @@ __global__ __launch_bounds__(256) void k_mgsb_pass(KrylovDims d, int p, const WI
 #pragma unroll
     for (int q = 0; q < NQ; ++q) acc[q] = cmk(0.0, 0.0);
     double nrm = 0.0, nav = 0.0;
-    // row groups in XCD-contiguous order: neighbouring groups share cache lines
-    // (a row of B slots need not be 128-byte aligned) and now share one L2
     if (m.ok)
-        mgsb::sweep_rows<BS, VT, WI, WO>(d, p, Win, Wout, V, Rhs, st, H, m.b,
-                                         xcd_remap(blockIdx.x, gridDim.x) * m.RW + m.rl, (long long)gridDim.x * m.RW,
-                                         acc, nrm, nav);
+        mgsb::sweep_rows<BS, VT, WI, WO>(d, p, Win, Wout, V, Rhs, st, H, m.b, (long long)blockIdx.x * m.RW + m.rl,
+                             (long long)gridDim.x * m.RW, acc, nrm, nav);
     snrm[threadIdx.x] = nrm;
     snav[threadIdx.x] = nav;
     __syncthreads();
@@ -588,7 +585,7 @@ __global__ __launch_bounds__(256) void k_vec_update(KrylovDims d, const SlotStat
     const int b = m.b;
     const SlotState s = st[b];
     const size_t nB = (size_t)d.n * d.B;
-    const long long r0 = xcd_remap(blockIdx.x, gridDim.x) * m.RW + m.rl, rs = (long long)gridDim.x * m.RW;
+    const long long r0 = (long long)blockIdx.x * m.RW + m.rl, rs = (long long)gridDim.x * m.RW;
     if (s.act == ACT_V0 || s.act == ACT_NEXT) {
         const double h = (s.act == ACT_V0) ? s.beta : s.hnorm;
         VT *vt = V + (size_t)s.tgt * nB;

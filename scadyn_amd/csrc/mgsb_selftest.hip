@@ -107,8 +107,7 @@ static void run_case(int B, int m, long long n, int nwg, std::mt19937_64 &rng, b
                     const bool ok = rl < RW && b < B;
                     for (int q = 0; q < NQ; ++q) acc[(size_t)t * NQ + q] = cmk(0.0, 0.0);
                     snrm[t] = snav[t] = 0.0;
-                    // rows as the kernel sweeps them (row groups in XCD-contiguous order)
-                    const long long r0 = xcd_remap(bx, nwg) * RW + rl, rs = (long long)nwg * RW;
+                    const long long r0 = (long long)bx * RW + rl, rs = (long long)nwg * RW;
                     cd *ac = &acc[(size_t)t * NQ];
                     if (ok && !wf)
                         sweep_rows<BS, VT, cd, cd>(d, p, W.data(), W.data(), V.data(), Rhs.data(), st.data(),
