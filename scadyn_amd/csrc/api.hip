@@ -787,10 +787,20 @@ static int auto_batch(scadyn_operator *op, int ncols) {
                 break;
             }
         }
+    // memory, not the column count, limits the batch
+    const bool capped = cap < (size_t)std::min(ncols, 128);
     B = (int)std::min<size_t>((size_t)B, cap);
     // a memory-capped batch keeps the slot-chunk kernels: round down to a width
     // whose last 64-slot chunk holds at least 48 slots
     if (B > 64 && B % 64 != 0 && B % 64 < 48) B -= B % 64;
+    // a capped batch whose last chunk is nearly full is completed to 64 slots
+    // when that still fits within 0.85 of the free memory: a full chunk runs
+    // each slot faster (824k-tet sphere, mixed: 64 slots 1.18 columns/s, 59
+    // slots 1.11)
+    if (capped && B % 64 >= 48) {
+        const int Bup = B + (64 - B % 64);
+        if (Bup <= ncols && (size_t)Bup * per <= (size_t)(0.85 * (double)free_b)) B = Bup;
+    }
     return std::max(1, B);
 }
 
