@@ -297,3 +297,19 @@ def test_matvec_dense_cubes_vs_oracle():
     assert srel < 5e-9
     assert wide < 1e-6 and narrow < 1e-6 and wide < 2 * narrow
     g.close()
+
+
+def test_tmatrix_leading_block_independent_of_nmax():
+    """The T-matrix of nmax 1 is the leading 3 x 3 part of each block of the
+    nmax-2 one: every column is its own solve and every row its own
+    projection, with index n(n+1)+m-1 and M-waves before N-waves (a wrong
+    column or row offset of the N-waves breaks it)."""
+    from scadyn_amd import native
+    c, e, param, k = sphere_case()
+    op = native.Operator(c, e, param, k, tol=1e-11, restart=40)
+    T1, _ = op.tmatrix(1)
+    T2, _ = op.tmatrix(2)
+    op.close()
+    worst = max(rel(b[:3, :3], a) for a, b in zip(T1, T2))
+    say(f'nmax 1 vs leading block of nmax 2: {worst:.2e}')
+    assert worst < 1e-9

@@ -165,3 +165,16 @@ def test_staircase_pfft_T_vs_mie():
     print('staircase 192/486 tets: Taa, Tbb errors', errs)
     assert errs[1][0] < errs[0][0]
     assert max(errs[1]) < 0.12 and errs[1][1] < 0.04
+
+
+def test_T_leading_block_independent_of_nmax():
+    """The oracle T of nmax 1 is the leading 3 x 3 part of each block of the
+    nmax-2 one (index n(n+1)+m-1, M-waves before N-waves in the block split)."""
+    from helpers import oracle_operator, sphere_case
+    from oracle import pfft_ref
+    c, e, param, k = sphere_case()
+    op = oracle_operator()
+    T1, _ = pfft_ref.tmatrix(op, 1, c, e, tol=1e-11, restart=20)
+    T2, _ = pfft_ref.tmatrix(op, 2, c, e, tol=1e-11, restart=20)
+    for a, b in zip(T1, T2):
+        assert np.linalg.norm(b[:3, :3] - a) <= 1e-12 * np.linalg.norm(a)
