@@ -49,6 +49,30 @@ module scadyn_amd_iface
          integer(c_int32_t), intent(out) :: iters(*)
          integer(c_int) :: rc
       end function
+
+      ! tr_T (src/mie.f90:728-759): crs = C_ext, C_sca, C_abs, and the trace sum
+      function scadyn_cross_sections(nmax, Taa, Tab, Tba, Tbb, k, crs) &
+         bind(C, name='scadyn_cross_sections') result(rc)
+         import :: c_double, c_int32_t, c_int, c_double_complex
+         integer(c_int32_t), value :: nmax
+         complex(c_double_complex), intent(in) :: Taa(*), Tab(*), Tba(*), Tbb(*)
+         real(c_double), value :: k
+         real(c_double), intent(out) :: crs(4)
+         integer(c_int) :: rc
+      end function
+
+      ! z components of forcetorque (src/forces.f90:81-257) for an x-polarised
+      ! plane wave along +z of amplitude E; a = volume-equivalent radius:
+      ! out = F_z (N), T_z (N m), Q_f, Q_t
+      function scadyn_force_torque_z(nmax, Taa, Tab, Tba, Tbb, k, E, a, out) &
+         bind(C, name='scadyn_force_torque_z') result(rc)
+         import :: c_double, c_int32_t, c_int, c_double_complex
+         integer(c_int32_t), value :: nmax
+         complex(c_double_complex), intent(in) :: Taa(*), Tab(*), Tba(*), Tbb(*)
+         real(c_double), value :: k, E, a
+         real(c_double), intent(out) :: out(4)
+         integer(c_int) :: rc
+      end function
    end interface
 
 contains

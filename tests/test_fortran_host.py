@@ -57,3 +57,9 @@ def test_fortran_host_tmatrix(tmp_path):
         if np.linalg.norm(b) > 0:
             assert rel(a, b) < 1e-7
     assert list(iters) == list(itref)
+    # the consumer called from Fortran gives the library's cross sections of the same T
+    from scadyn_amd import native
+    line = [ln for ln in out.splitlines() if ln.strip().startswith('cross sections')][0]
+    crs_f = np.array([float(v) for v in line.split()[2:5]])
+    crs = native.cross_sections(T, k)[:3]
+    assert np.allclose(crs_f, crs, rtol=1e-12, atol=0.0), (crs_f, crs)
