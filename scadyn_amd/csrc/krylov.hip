@@ -587,27 +587,13 @@ __global__ __launch_bounds__(256) void k_vec_update(KrylovDims d, const SlotStat
     const size_t nB = (size_t)d.n * d.B;
     const long long r0 = (long long)blockIdx.x * m.RW + m.rl, rs = (long long)gridDim.x * m.RW;
     if (s.act == ACT_V0 || s.act == ACT_NEXT) {
-        // v = w / h (w unchanged when h = 0: division by 1 is exact)
-        const double h0 = (s.act == ACT_V0) ? s.beta : s.hnorm;
-        const double h = h0 != 0.0 ? h0 : 1.0;
+        const double h = (s.act == ACT_V0) ? s.beta : s.hnorm;
         VT *vt = V + (size_t)s.tgt * nB;
-        // four rows' loads are issued before the first store, so that a wave
-        // has four loads in flight instead of waiting on each
-        long long row = r0;
-        for (; row + 3 * rs < d.n; row += 4 * rs) {
-            WT wr[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) wr[u] = W[(size_t)(row + u * rs) * d.B + b];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const cd w = Prec<cd>::from(wr[u]);
-                vt[(size_t)(row + u * rs) * d.B + b] = Prec<VT>::from(cmk(w.x / h, w.y / h));
-            }
-        }
-        for (; row < d.n; row += rs) {
+        for (long long row = r0; row < d.n; row += rs) {
             const size_t e = (size_t)row * d.B + b;
-            const cd w = Prec<cd>::from(W[e]);
-            vt[e] = Prec<VT>::from(cmk(w.x / h, w.y / h));
+            cd w = Prec<cd>::from(W[e]);
+            if (h != 0.0) w = cmk(w.x / h, w.y / h);
+            vt[e] = Prec<VT>::from(w);
         }
     } else if (s.act == ACT_XUPD) {
         const cd *y = yall + (size_t)b * d.m;
