@@ -69,14 +69,11 @@ def say(msg):
 # scaled by the copy-kernel calibration factor (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0):
 # (workload, kernel class, precision, batch) -> bytes
 MEASURED_TRAFFIC = {
-    # profiles/pmc_traffic_matvec_layered824k.md (the bench batches: mixed 64, FP64 43;
-    # mixed 59 before a nearly full memory-capped batch was completed to 64 slots)
+    # profiles/pmc_traffic_matvec_layered824k.md (the bench batches: mixed 64, FP64 43)
     ('layered824k', 'fftz_conv', 'mixed', 64): 2.0 * 9662131 * 1024 + 19245600 * 1024,
     ('layered824k', 'gather_near', 'mixed', 64): 2.0 * 17700921 * 1024 + 2472768 * 1024,
-    ('layered824k', 'fftz_conv', 'mixed', 59): 2.0 * 9126410 * 1024 + 17742038 * 1024,
-    ('layered824k', 'gather_near', 'mixed', 59): 2.0 * 16326260 * 1024 + 2281980 * 1024,
-    ('layered824k', 'fftz_conv', 'fp64', 43): 2.0 * 13473551 * 1024 + 25861275 * 1024,
-    ('layered824k', 'gather_near', 'fp64', 43): 2.0 * 20644424 * 1024 + 1663746 * 1024,
+    ('layered824k', 'fftz_conv', 'fp64', 43): 2.0 * 13006873 * 1024 + 25861275 * 1024,
+    ('layered824k', 'gather_near', 'fp64', 43): 2.0 * 20667058 * 1024 + 1663739 * 1024,
     # profiles/pmc_traffic_matvec_ellipsoid200k_b112.md
     ('ellipsoid200k', 'gather_near', 'fp64', 112): 28.08e9,
     ('ellipsoid200k', 'fftz_conv', 'fp64', 112): 39.17e9,
