@@ -55,6 +55,8 @@ sys.path.insert(0, ROOT)
 METRIC = 'T-matrix columns/s, mixed vs FP64 pFFT-GMRES at 1e-6 rel err; p99 latency'
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFLOPS = 78.6
+# scadyn_opts.basis values by name (storage of the mixed-precision Krylov basis)
+BASES = {'auto': 0, 'c64': 1, 'f16': 2}
 MV_CLASSES = ['scatter', 'fftx', 'ffty', 'fftz_conv', 'iffty', 'ifftx', 'gather_near']
 
 from scadyn_amd.workloads import make_workload, request_stream  # noqa: E402
@@ -166,7 +168,7 @@ def headline(args):
         t = time.time()
         if name == 'mixed':
             op = native.Operator(c, e, param, k, precision=native.PREC_MIXED, tol=args.inner_tol,
-                                 refine_tol=args.tol, **common)
+                                 refine_tol=args.tol, basis=BASES[args.basis], **common)
             warm, steps = args.warmup, args.steps
         else:
             op = native.Operator(c, e, param, k, precision=native.PREC_FP64, tol=args.tol, **common)
@@ -199,7 +201,7 @@ def headline(args):
     cfg = {'workload': desc, 'n_tet': int(e.shape[0]), 'nmax': nmax, 'columns_total': ncol,
            'columns_per_step': args.cols_per_step, 'column_order': 'seeded random permutation (seed 0)',
            'gmres': {'tol': args.tol, 'mixed_cycle_tol': args.inner_tol, 'maxit': args.maxit,
-                     'restart': args.restart},
+                     'restart': args.restart, 'mixed_basis': args.basis},
            'mixed': {kk: v for kk, v in head.items() if kk != 'roofline'},
            'fp64': legs['fp64'],
            'speedup_mixed_over_fp64': head['columns_per_s'] / legs['fp64']['columns_per_s'],
@@ -288,7 +290,7 @@ def probe(args):
     prec = native.PREC_MIXED if args.precision == 'mixed' else native.PREC_FP64
     say(f'{args.workload}: {e.shape[0]} tets ({desc})')
     op = native.Operator(c, e, param, k, batch=args.batch or 64, precision=prec, tol=args.tol,
-                         maxit=args.maxit, restart=args.restart)
+                         maxit=args.maxit, restart=args.restart, basis=BASES[args.basis])
     if args.matvecs:
         B = op.opts.batch
         op.profile(True)
@@ -331,6 +333,8 @@ def main():
     ap.add_argument('--inner-tol', type=float, default=1e-4,
                     help='mixed: residual reduction that ends a restart cycle early')
     ap.add_argument('--maxit', type=int, default=50, help='Krylov dimension per cycle (reference default 50)')
+    ap.add_argument('--basis', choices=sorted(BASES), default='auto',
+                    help='mixed: storage of the Krylov basis (auto: binary16 up to 2^26 rows)')
     ap.add_argument('--mgs-block', type=int, default=0,
                     help='MGS coefficients per pass over the rows (0 automatic; 1, 2, 4, 6 or 8)')
     ap.add_argument('--restart', type=int, default=60, help='restart cycles (enough for every column to converge)')

@@ -46,9 +46,14 @@ extern "C" {
 #define SCADYN_ERR_UNSUPPORTED  8  /* option the build does not implement            */
 
 #define SCADYN_PREC_FP64   0  /* FP64 matvec, reference-faithful GMRES          */
-#define SCADYN_PREC_MIXED  1  /* FP32 matvec (and, with the blocked MGS, an FP32-stored
-                                 Krylov basis with FP64 arithmetic) inside FP64 iterative
-                                 refinement                                          */
+#define SCADYN_PREC_MIXED  1  /* FP32 matvec (and, with the blocked MGS, a Krylov basis
+                                 stored in reduced precision with FP64 arithmetic)
+                                 inside FP64 iterative refinement                    */
+
+/* storage of the mixed-precision Krylov basis (scadyn_opts.basis) */
+#define SCADYN_BASIS_AUTO  0
+#define SCADYN_BASIS_C64   1
+#define SCADYN_BASIS_F16   2
 
 typedef struct scadyn_mesh {
     int32_t n_node;
@@ -94,7 +99,15 @@ typedef struct scadyn_opts {
     int32_t gather_kernel; /* complex64 gather + near zone: 0 vector FMA (default), 1 the near-zone
                               product on the matrix cores (v_mfma_f32_16x16x4f32; measured about 2x
                               slower, its operand set-up costs as much as the FMAs it replaces) */
-    int32_t reserved[4];
+    int32_t basis;       /* mixed precision with the blocked MGS (ortho 0, mgs_block > 1):
+                            storage of the Krylov basis.  SCADYN_BASIS_AUTO (binary16 where
+                            the system has at most 2^26 rows, else complex64),
+                            SCADYN_BASIS_C64, or SCADYN_BASIS_F16: real and imaginary parts
+                            in IEEE binary16, half the basis traffic of complex64.  The
+                            arithmetic stays FP64 and every restart checks the FP64 true
+                            residual, so the stored precision changes iteration counts,
+                            not the accuracy of the result                              */
+    int32_t reserved[3];
 } scadyn_opts;
 
 /* Defaults of the reference (src/common.f90:78-89). */

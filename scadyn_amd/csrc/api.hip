@@ -146,11 +146,13 @@ struct Workspace {
     int *rslot = nullptr;
     // Krylov basis: complex128 V [(m+2)][n][B] with x in block m+1 (X points
     // there), or for the mixed-precision solve (ortho 0, mgs_block > 1) a
-    // complex64 basis Vf [(m+1)][n][B] and x in its own complex128 X
+    // complex64 basis Vf or a binary16 one Vh [(m+1)][n][B] and x in its own
+    // complex128 X
     cd *V = nullptr, *Rhs = nullptr, *W = nullptr, *X = nullptr;
     cf *Vf = nullptr, *Wf = nullptr;  // complex64 basis and orthogonalised w (mixed precision)
-    // mixed precision with a complex128 basis: the batch's inputs converted to
-    // complex64 (Xs [n][B], read with zero offsets zoff)
+    ch *Vh = nullptr;                 // binary16 basis (mixed precision; Xs holds the matvec input)
+    // mixed precision with a complex128 or binary16 basis: the batch's inputs
+    // converted to complex64 (Xs [n][B], read with zero offsets zoff)
     cf *Xs = nullptr;
     long long *zoff = nullptr;
     int *ract = nullptr;  // mixed precision: restarting slots of a full-width FP64 matvec
@@ -241,6 +243,16 @@ constexpr int MGS_BLOCK_AUTO = 4;
 static bool f32_basis(const scadyn_operator *op) {
     const int bs = op->opts.mgs_block == 0 ? MGS_BLOCK_AUTO : op->opts.mgs_block;
     return op->opts.precision == SCADYN_PREC_MIXED && op->opts.ortho == 0 && bs > 1;
+}
+// Of those, the ones whose basis is stored in binary16.  A unit basis vector
+// of n rows rounds with a relative error below 2^-11 per element; elements
+// under binary16's normal range (2^-14) add at most sqrt(n) 2^-25 to the
+// vector's norm error, which stays below 2^-12 while n <= 2^26.
+constexpr long long F16_BASIS_MAX_ROWS = 1LL << 26;
+static bool f16_basis(const scadyn_operator *op) {
+    if (!f32_basis(op)) return false;
+    if (op->opts.basis == SCADYN_BASIS_F16) return true;
+    return op->opts.basis == SCADYN_BASIS_AUTO && 3LL * op->g.N <= F16_BASIS_MAX_ROWS;
 }
 
 // xs[row*B + b] = (complex64) v[in_off[b] + row*B + b] for active slots
@@ -451,10 +463,12 @@ static void create_operator(scadyn_operator *op, const scadyn_mesh &mesh, const 
 static size_t slot_bytes(const scadyn_operator *op, int m) {
     const size_t n = 3 * (size_t)op->g.N;
     const size_t grid = (size_t)op->L[0] * op->L[1] * op->g.Nz * 4;
-    // complex64 basis Vf and w copy with complex128 x, W, Rhs; or the complex128
+    // complex64 basis Vf and w copy with complex128 x, W, Rhs; the binary16 basis
+    // Vh with the same and the complex64 matvec input Xs; or the complex128
     // basis with x, W, Rhs and one vector of slack
-    const size_t vec = f32_basis(op) ? (size_t)(m + 1) * n * sizeof(cf) + n * (3 * sizeof(cd) + sizeof(cf))
-                                     : (size_t)(m + 4) * n * sizeof(cd);
+    const size_t vec = f16_basis(op)   ? (size_t)(m + 1) * n * sizeof(ch) + n * (3 * sizeof(cd) + 2 * sizeof(cf))
+                       : f32_basis(op) ? (size_t)(m + 1) * n * sizeof(cf) + n * (3 * sizeof(cd) + sizeof(cf))
+                                       : (size_t)(m + 4) * n * sizeof(cd);
     return vec + 2 * grid * sizeof(cd);  // two grids (complex128 size: both precisions)
 }
 
@@ -493,7 +507,12 @@ static Workspace &workspace(scadyn_operator *op, int B) {
     const size_t nB = (size_t)w.n * B;
     make_matvec_space(op, B, w.mv);
     tune_fft_tiles(op, w.mv);
-    if (f32_basis(op)) {
+    if (f16_basis(op)) {
+        w.Vh = w.mem.alloc<ch>((size_t)(m + 1) * nB);
+        w.Wf = w.mem.alloc<cf>(nB);
+        w.X = w.mem.alloc<cd>(nB);
+        w.Xs = w.mem.alloc<cf>(nB);
+    } else if (f32_basis(op)) {
         w.Vf = w.mem.alloc<cf>((size_t)(m + 1) * nB);
         w.Wf = w.mem.alloc<cf>(nB);
         w.X = w.mem.alloc<cd>(nB);
@@ -886,9 +905,9 @@ struct ColumnSolver {
         spec = sp;
         B = auto_batch(op, (int)cols.size());
         w = &workspace(op, B);
-        // a complex64 basis serves only the mixed-precision blocked-MGS solve
-        if (w->Vf && (spec.prec != SCADYN_PREC_MIXED || spec.ortho != 0 || spec.mgs_block <= 1))
-            fail(SCADYN_ERR_ARG, "solve settings do not match the operator's complex64 Krylov basis");
+        // a complex64 or binary16 basis serves only the mixed-precision blocked-MGS solve
+        if ((w->Vf || w->Vh) && (spec.prec != SCADYN_PREC_MIXED || spec.ortho != 0 || spec.mgs_block <= 1))
+            fail(SCADYN_ERR_ARG, "solve settings do not match the operator's reduced-precision Krylov basis");
         d.n = w->n;
         d.B = B;
         d.m = w->m;
@@ -909,7 +928,10 @@ struct ColumnSolver {
         const size_t nB = (size_t)w->n * B;
         HIP_CHECK(hipMemsetAsync(w->st, 0, sizeof(SlotState) * B, op->s));
         HIP_CHECK(hipMemsetAsync(w->ierr, 0, sizeof(int), op->s));  // the workspace may be reused
-        if (w->Vf) {
+        if (w->Vh) {
+            HIP_CHECK(hipMemsetAsync(w->Vh, 0, sizeof(ch) * (size_t)(w->m + 1) * nB, op->s));
+            HIP_CHECK(hipMemsetAsync(w->X, 0, sizeof(cd) * nB, op->s));
+        } else if (w->Vf) {
             HIP_CHECK(hipMemsetAsync(w->Vf, 0, sizeof(cf) * (size_t)(w->m + 1) * nB, op->s));
             HIP_CHECK(hipMemsetAsync(w->X, 0, sizeof(cd) * nB, op->s));
         } else {
@@ -986,9 +1008,15 @@ struct ColumnSolver {
                 if (hs.p[b].phase == PH_ARNOLDI || (hs.p[b].phase == PH_RESTART && !spec.verify64)) ++narn;
             }
             launch_set_inputs(d, w->st, w->mv.in_off, w->mv.active, spec.verify64 ? 1 : 0,
-                              w->Vf ? w->m + 1 : w->m + 2, w->ierr, s);
+                              (w->Vf || w->Vh) ? w->m + 1 : w->m + 2, w->ierr, s);
             if (narn > 0) {
-                if (spec.prec == SCADYN_PREC_MIXED && w->Vf) {
+                if (spec.prec == SCADYN_PREC_MIXED && w->Vh) {
+                    {
+                        ProfScope pk(op, SCADYN_KCLASS_KRYLOV, 0.0);
+                        launch_stage_basis(d, w->Vh, w->mv.in_off, w->mv.active, w->Xs, s);
+                    }
+                    run_matvec<cf>(op, w->mv, w->Xs, w->mv.active, w->W, w->zoff);
+                } else if (spec.prec == SCADYN_PREC_MIXED && w->Vf) {
                     run_matvec<cf>(op, w->mv, w->Vf, w->mv.active, w->W);
                 } else if (spec.prec == SCADYN_PREC_MIXED) {
                     launch_stage_f32(w->V, w->mv.in_off, w->mv.active, w->n, B, w->Xs, s);
@@ -1022,6 +1050,9 @@ struct ColumnSolver {
             } else if (spec.ortho == 2) {
                 launch_bmgs(d, w->W, w->V, w->Rhs, w->st, w->H, w->pdot, w->pnrm, w->pnav, w->cc, w->nrm2, w->nav2,
                             std::max(K - 1, 0), s);
+            } else if (spec.mgs_block > 1 && w->Vh) {
+                launch_mgsb(d, spec.mgs_block, K, w->W, w->Wf, (const ch *)w->Vh, w->Rhs, w->st, w->H, w->pdot,
+                            w->pnrm, w->pnav, w->nrm2, w->nav2, s);
             } else if (spec.mgs_block > 1 && w->Vf) {
                 launch_mgsb(d, spec.mgs_block, K, w->W, w->Wf, (const cf *)w->Vf, w->Rhs, w->st, w->H, w->pdot,
                             w->pnrm, w->pnav, w->nrm2, w->nav2, s);
@@ -1036,7 +1067,9 @@ struct ColumnSolver {
             }
             launch_slot_update(d, w->st, w->H, w->cs, w->sn, w->g, w->y, w->nrm2, w->nav2, spec.tol, spec.restart,
                                spec.verify64 ? 1 : 0, spec.tol_true, spec.align ? 1 : 0, s);
-            if (w->Vf)
+            if (w->Vh)
+                launch_vec_update(d, w->st, (const cf *)w->Wf, w->Vh, w->X, w->y, s);
+            else if (w->Vf)
                 launch_vec_update(d, w->st, (const cf *)w->Wf, w->Vf, w->X, w->y, s);
             else
                 launch_vec_update(d, w->st, w->W, w->V, w->X, w->y, s);
@@ -1604,6 +1637,8 @@ int scadyn_op_create(const scadyn_mesh *mesh, const scadyn_opts *opts, double k,
     if (o.fft_kernel != 0 && o.fft_kernel != 1) fail(SCADYN_ERR_ARG, "fft_kernel must be 0 (automatic) or 1 (Stockham)");
     if (o.gather_kernel != 0 && o.gather_kernel != 1)
         fail(SCADYN_ERR_ARG, "gather_kernel must be 0 (vector FMA) or 1 (matrix cores)");
+    if (o.basis != SCADYN_BASIS_AUTO && o.basis != SCADYN_BASIS_C64 && o.basis != SCADYN_BASIS_F16)
+        fail(SCADYN_ERR_ARG, "basis must be SCADYN_BASIS_AUTO, SCADYN_BASIS_C64 or SCADYN_BASIS_F16");
     if (o.mgs_block != 0 && o.mgs_block != 1 && o.mgs_block != 2 && o.mgs_block != 4 && o.mgs_block != 6 &&
         o.mgs_block != 8)
         fail(SCADYN_ERR_ARG, "mgs_block must be 0 (automatic), 1, 2, 4, 6 or 8");

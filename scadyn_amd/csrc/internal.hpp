@@ -31,6 +31,11 @@ void set_error(int code, const std::string &msg);
 // ---------------------------------------------------------------- complex
 typedef double2 cd;
 typedef float2 cf;
+// complex with IEEE binary16 parts: the storage of a reduced-precision Krylov
+// basis (never computed in; converted through float on every load and store)
+struct __attribute__((aligned(4))) ch {
+    _Float16 x, y;
+};
 
 // Workgroup b of a G-workgroup launch -> work item, a bijection of [0, G).
 // Workgroups are dealt to the 8 XCDs round-robin (b % 8), each XCD with its own
@@ -88,6 +93,7 @@ struct Prec<cd> {
     typedef double real;
     __host__ __device__ static cd from(cd a) { return a; }
     __host__ __device__ static cd from(cf a) { return make_double2(a.x, a.y); }
+    __host__ __device__ static cd from(ch a) { return make_double2((double)(float)a.x, (double)(float)a.y); }
     __host__ __device__ static cd zero() { return make_double2(0.0, 0.0); }
 };
 template <>
@@ -95,7 +101,17 @@ struct Prec<cf> {
     typedef float real;
     __host__ __device__ static cf from(cd a) { return make_float2((float)a.x, (float)a.y); }
     __host__ __device__ static cf from(cf a) { return a; }
+    __host__ __device__ static cf from(ch a) { return make_float2((float)a.x, (float)a.y); }
     __host__ __device__ static cf zero() { return make_float2(0.0f, 0.0f); }
+};
+template <>
+struct Prec<ch> {
+    typedef float real;  // arithmetic on a stored element runs in float or double
+    // double -> float -> binary16, both rounding to nearest even: the same
+    // result on the host and the device
+    __host__ __device__ static ch from(cd a) { return {(_Float16)(float)a.x, (_Float16)(float)a.y}; }
+    __host__ __device__ static ch from(cf a) { return {(_Float16)a.x, (_Float16)a.y}; }
+    __host__ __device__ static ch zero() { return {(_Float16)0.0f, (_Float16)0.0f}; }
 };
 
 // ---------------------------------------------------------------- geometry (host)

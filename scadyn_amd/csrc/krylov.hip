@@ -255,6 +255,10 @@ void launch_mgsb(const KrylovDims &d, int bs, int K, cd *W, cf *Wf, const cf *V,
                  cd *H, cd *pdot, double *pnrm, double *pnav, double *nrm2, double *nav2, hipStream_t s) {
     launch_mgsb_v<cf>(d, bs, K, W, Wf, V, Rhs, st, H, pdot, pnrm, pnav, nrm2, nav2, s);
 }
+void launch_mgsb(const KrylovDims &d, int bs, int K, cd *W, cf *Wf, const ch *V, const cd *Rhs, const SlotState *st,
+                 cd *H, cd *pdot, double *pnrm, double *pnav, double *nrm2, double *nav2, hipStream_t s) {
+    launch_mgsb_v<ch>(d, bs, K, W, Wf, V, Rhs, st, H, pdot, pnrm, pnav, nrm2, nav2, s);
+}
 int mgsb_partials(int bs) { return bs + bs * (bs - 1) / 2; }
 
 // ---- block modified Gram-Schmidt (opts.ortho = 2) ------------------------------
@@ -850,6 +854,31 @@ void launch_vec_update(const KrylovDims &d, const SlotState *st, const cd *W, cd
 void launch_vec_update(const KrylovDims &d, const SlotState *st, const cf *W, cf *V, cd *X, const cd *y,
                        hipStream_t s) {
     launch_vec_update_t<cf, cf>(d, st, W, V, X, y, s);
+}
+void launch_vec_update(const KrylovDims &d, const SlotState *st, const cf *W, ch *V, cd *X, const cd *y,
+                       hipStream_t s) {
+    launch_vec_update_t<ch, cf>(d, st, W, V, X, y, s);
+}
+
+// xs[row B + b] = (complex64) V[in_off[b] + row B + b] for active slots b, the
+// rows strided as in the sweeps
+__global__ __launch_bounds__(256) void k_stage_basis(KrylovDims d, const ch *__restrict__ V,
+                                                     const long long *__restrict__ in_off,
+                                                     const int *__restrict__ active, cf *__restrict__ xs) {
+    const RowSlotMap m = row_slot_map(d.B);
+    if (!m.ok || !active[m.b]) return;
+    const ch *v = V + in_off[m.b];
+    for (long long row = (long long)blockIdx.x * m.RW + m.rl; row < d.n; row += (long long)gridDim.x * m.RW) {
+        const size_t e = (size_t)row * d.B + m.b;
+        xs[e] = Prec<cf>::from(v[e]);
+    }
+}
+void launch_stage_basis(const KrylovDims &d, const ch *V, const long long *in_off, const int *active, cf *xs,
+                        hipStream_t s) {
+    const int SB = d.B < 256 ? d.B : 256;
+    const dim3 grid((unsigned)d.nwg, (unsigned)((d.B + SB - 1) / SB));
+    hipLaunchKernelGGL(k_stage_basis, grid, dim3(256), 0, s, d, V, in_off, active, xs);
+    HIP_CHECK(hipGetLastError());
 }
 void launch_load_rhs(const KrylovDims &d, const int *col, const cd *src, long long ld, cd *Rhs, cd *X,
                      double *pnrm, int x0zero, hipStream_t s) {

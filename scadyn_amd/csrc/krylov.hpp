@@ -55,6 +55,12 @@ void launch_vec_update(const KrylovDims &d, const SlotState *st, const cd *W, cd
                        hipStream_t s);
 void launch_vec_update(const KrylovDims &d, const SlotState *st, const cf *W, cf *V, cd *X, const cd *y,
                        hipStream_t s);
+void launch_vec_update(const KrylovDims &d, const SlotState *st, const cf *W, ch *V, cd *X, const cd *y,
+                       hipStream_t s);
+// the matvec inputs of a binary16 basis (block in_off[b] of V for active slots)
+// converted to complex64 in xs (n x B)
+void launch_stage_basis(const KrylovDims &d, const ch *V, const long long *in_off, const int *active, cf *xs,
+                        hipStream_t s);
 // load column col[b] (>= 0) of src (n x ld, row-major) into Rhs and X of slot b
 void launch_load_rhs(const KrylovDims &d, const int *col, const cd *src, long long ld, cd *Rhs, cd *X,
                      double *pnrm, int x0zero, hipStream_t s);
@@ -84,6 +90,9 @@ void launch_mgsb(const KrylovDims &d, int bs, int K, cd *W, const cd *V, const c
 // mixed precision: complex64 basis, and w kept in the complex64 Wf once the
 // first pass that changes it has run (read from the complex128 W before)
 void launch_mgsb(const KrylovDims &d, int bs, int K, cd *W, cf *Wf, const cf *V, const cd *Rhs, const SlotState *st,
+                 cd *H, cd *pdot, double *pnrm, double *pnav, double *nrm2, double *nav2, hipStream_t s);
+// the same with the basis in binary16 (ch)
+void launch_mgsb(const KrylovDims &d, int bs, int K, cd *W, cf *Wf, const ch *V, const cd *Rhs, const SlotState *st,
                  cd *H, cd *pdot, double *pnrm, double *pnav, double *nrm2, double *nav2, hipStream_t s);
 int mgsb_partials(int bs);
 

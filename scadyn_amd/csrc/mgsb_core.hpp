@@ -60,8 +60,8 @@ MGSB_HD PassRole pass_role(const SlotState &s, int p) {
 
 // one thread's rows (row0, row0 + rstep, ...) of slot b: partial dots/Gram in
 // acc[NQ], |w|^2 after the pass's update in nrm, before orthogonalisation in nav.
-// The basis is stored as VT (complex128, or complex64 for the mixed-precision
-// solve); all arithmetic is FP64.  w is read from Win (WI) and, where the pass
+// The basis is stored as VT (complex128, or complex64 or binary16 complex for
+// the mixed-precision solve); all arithmetic is FP64.  w is read from Win (WI) and, where the pass
 // changes it, written to Wout (WO): the same complex128 array, or for the
 // mixed-precision solve the complex128 matvec output in, a complex64 copy out
 // from the first pass that writes w on (passes 0 and 1), then the copy alone.
@@ -96,7 +96,7 @@ MGSB_HD void sweep_rows(const KrylovDims &d, int p, const WI *Win, WO *Wout, con
         // by one).  With the complex128 basis only the dot vectors go ahead
         // and the update vectors are loaded at their use: all of them up front
         // take 140 VGPRs (3 waves per SIMD instead of 4), which costs more.
-        constexpr bool AHEAD = sizeof(VT) == 8;
+        constexpr bool AHEAD = sizeof(VT) <= 8;
         const WI wraw = Win[e];
         VT vu[BS], vd[BS];
         if constexpr (AHEAD) {

@@ -51,7 +51,7 @@ static void run_case(int B, int m, long long n, int nwg, std::mt19937_64 &rng, b
     const size_t nB = (size_t)n * B;
     std::normal_distribution<double> nd;
     auto rnd = [&]() { return cmk(nd(rng), nd(rng)); };
-    std::vector<VT> V((size_t)(m + 1) * nB);  // the basis as stored (complex128 or complex64)
+    std::vector<VT> V((size_t)(m + 1) * nB);  // the basis as stored (complex128, complex64 or binary16)
     std::vector<cd> W(nB), Rhs(nB), H((size_t)B * (m + 1) * m);
     std::vector<cf> Wf(wf ? nB : 0, cmkf(1e30f, 1e30f));  // written before it is read
     for (auto &z : V) z = Prec<VT>::from(rnd());
@@ -206,7 +206,7 @@ static void run_case(int B, int m, long long n, int nwg, std::mt19937_64 &rng, b
     const bool ok = worst < tol;
     if (!ok) ++failures;
     std::printf("BS=%d %s%s B=%3d m=%2d n=%5lld nwg=%4d passes=%2d  worst rel.err %.2e %s\n", BS,
-                sizeof(VT) == 16 ? "c128" : "c64 ", wf ? " w64" : "    ", B, m, n, nwg, P + 1, worst,
+                sizeof(VT) == 16 ? "c128" : sizeof(VT) == 8 ? "c64 " : "f16 ", wf ? " w64" : "    ", B, m, n, nwg, P + 1, worst,
                 ok ? "ok" : "FAIL");
 }
 
@@ -252,6 +252,8 @@ int main() {
             run_case<8, cf>(B, m, n, 7, rng);
             run_case<4, cf>(B, m, n, nwg, rng, true);  // ... with complex64 w
             run_case<2, cf>(B, m, n, 7, rng, true);
+            run_case<4, ch>(B, m, n, nwg, rng, true);  // binary16 basis with complex64 w
+            run_case<8, ch>(B, m, n, 7, rng, true);
         }
     std::printf("%d failures\n", failures);
     return failures ? 1 : 0;

@@ -23,8 +23,8 @@ module scadyn_amd_iface
       real(c_double) :: grid_size
       integer(c_int32_t) :: order, precision, batch, strict, device, fft_pad
       real(c_double) :: refine_tol
-      integer(c_int32_t) :: ortho, fft_kernel, mgs_block, gather_kernel
-      integer(c_int32_t) :: reserved(4)
+      integer(c_int32_t) :: ortho, fft_kernel, mgs_block, gather_kernel, basis
+      integer(c_int32_t) :: reserved(3)
    end type
 
    interface

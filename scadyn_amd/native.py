@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, 'libscadyn_amd.so')
 OK = 0
 ERR_ARG, ERR_ORIENTATION, ERR_GRID, ERR_SINGULAR, ERR_NOCONV, ERR_MEMORY, ERR_DEVICE, ERR_UNSUPPORTED = range(1, 9)
 PREC_FP64, PREC_MIXED = 0, 1
+BASIS_AUTO, BASIS_C64, BASIS_F16 = 0, 1, 2  # storage of the mixed-precision Krylov basis
 EXPORT_S, EXPORT_BLOCKS, EXPORT_BLKIND, EXPORT_FG = 1, 2, 3, 4
 KCLASS = ['scatter', 'fftx', 'ffty', 'fftz_conv', 'iffty', 'ifftx', 'gather_near', 'krylov', 'tgemm']
 
@@ -54,7 +55,8 @@ class COpts(ctypes.Structure):
                 ('order', ctypes.c_int32), ('precision', ctypes.c_int32), ('batch', ctypes.c_int32),
                 ('strict', ctypes.c_int32), ('device', ctypes.c_int32), ('fft_pad', ctypes.c_int32),
                 ('refine_tol', ctypes.c_double), ('ortho', ctypes.c_int32), ('fft_kernel', ctypes.c_int32),
-                ('mgs_block', ctypes.c_int32), ('gather_kernel', ctypes.c_int32), ('reserved', ctypes.c_int32 * 4)]
+                ('mgs_block', ctypes.c_int32), ('gather_kernel', ctypes.c_int32), ('basis', ctypes.c_int32),
+                ('reserved', ctypes.c_int32 * 3)]
 
 
 _lib = None

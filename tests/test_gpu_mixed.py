@@ -115,3 +115,29 @@ def test_mixed_desynchronised_restarts(case):
         assert rel(xm[r], x64[r]) < 1e-8
     op64.close()
     opm.close()
+
+
+def test_mixed_basis_storage(case):
+    """The mixed solve with its Krylov basis stored in binary16 and in complex64
+    (scadyn_opts.basis): both within 1e-8 of a tight FP64 solve, since the FP64
+    true residual at every restart decides convergence, and the binary16 basis
+    takes at most 1.5x the matvecs of the complex64 one."""
+    from scadyn_amd import native
+    c, e, param, k = case
+    rng = np.random.default_rng(12)
+    b = rng.standard_normal((6, 3 * e.shape[0])) + 1j * rng.standard_normal((6, 3 * e.shape[0]))
+    say('fp64 tight, 6 systems')
+    op64 = native.Operator(c, e, param, k, tol=1e-11, restart=20)
+    x64, _ = op64.solve(b)
+    its = {}
+    for name, basis in (('complex64', native.BASIS_C64), ('binary16', native.BASIS_F16)):
+        opm = native.Operator(c, e, param, k, precision=native.PREC_MIXED, tol=1e-4, refine_tol=1e-10,
+                              restart=60, basis=basis)
+        xm, it = opm.solve(b)
+        its[name] = int(it.sum())
+        say(f'{name} basis: matvecs {it.tolist()}')
+        for r in range(6):
+            assert rel(xm[r], x64[r]) < 1e-8
+        opm.close()
+    assert its['binary16'] <= 1.5 * its['complex64']
+    op64.close()
