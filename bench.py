@@ -334,7 +334,7 @@ def main():
                     help='mixed: residual reduction that ends a restart cycle early')
     ap.add_argument('--maxit', type=int, default=50, help='Krylov dimension per cycle (reference default 50)')
     ap.add_argument('--basis', choices=sorted(BASES), default='auto',
-                    help='mixed: storage of the Krylov basis (auto: binary16 up to 2^26 rows)')
+                    help='mixed: storage of the Krylov basis (auto: binary16 for 2^20 to 2^26 rows)')
     ap.add_argument('--mgs-block', type=int, default=0,
                     help='MGS coefficients per pass over the rows (0 automatic; 1, 2, 4, 6 or 8)')
     ap.add_argument('--restart', type=int, default=60, help='restart cycles (enough for every column to converge)')

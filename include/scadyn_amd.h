@@ -100,13 +100,14 @@ typedef struct scadyn_opts {
                               product on the matrix cores (v_mfma_f32_16x16x4f32; measured about 2x
                               slower, its operand set-up costs as much as the FMAs it replaces) */
     int32_t basis;       /* mixed precision with the blocked MGS (ortho 0, mgs_block > 1):
-                            storage of the Krylov basis.  SCADYN_BASIS_AUTO (binary16 where
-                            the system has at most 2^26 rows, else complex64),
+                            storage of the Krylov basis.  SCADYN_BASIS_AUTO (binary16 for
+                            systems of 2^20 to 2^26 rows, else complex64),
                             SCADYN_BASIS_C64, or SCADYN_BASIS_F16: real and imaginary parts
                             in IEEE binary16, half the basis traffic of complex64.  The
                             arithmetic stays FP64 and every restart checks the FP64 true
                             residual, so the stored precision changes iteration counts,
-                            not the accuracy of the result                              */
+                            not the accuracy of the result.  With binary16 a restart
+                            cycle ends at a residual reduction of max(tol, 2^-10)      */
     int32_t reserved[3];
 } scadyn_opts;
 

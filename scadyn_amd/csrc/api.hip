@@ -247,12 +247,20 @@ static bool f32_basis(const scadyn_operator *op) {
 // Of those, the ones whose basis is stored in binary16.  A unit basis vector
 // of n rows rounds with a relative error below 2^-11 per element; elements
 // under binary16's normal range (2^-14) add at most sqrt(n) 2^-25 to the
-// vector's norm error, which stays below 2^-12 while n <= 2^26.
+// vector's norm error, which stays below 2^-12 while n <= 2^26.  The rounding
+// limits what one restart cycle can reduce the residual by to about 2^-10, so
+// such a cycle ends at that reduction at the latest (F16_CYCLE_TOL): a cycle
+// asked for more would run on without gaining it.  The automatic choice takes
+// binary16 from 2^20 rows, where the basis sweeps dominate the memory traffic;
+// smaller systems, which often converge in a few short cycles, keep complex64.
 constexpr long long F16_BASIS_MAX_ROWS = 1LL << 26;
+constexpr long long F16_BASIS_AUTO_MIN_ROWS = 1LL << 20;
+constexpr double F16_CYCLE_TOL = 1.0 / 1024.0;
 static bool f16_basis(const scadyn_operator *op) {
     if (!f32_basis(op)) return false;
     if (op->opts.basis == SCADYN_BASIS_F16) return true;
-    return op->opts.basis == SCADYN_BASIS_AUTO && 3LL * op->g.N <= F16_BASIS_MAX_ROWS;
+    const long long n = 3LL * op->g.N;
+    return op->opts.basis == SCADYN_BASIS_AUTO && n >= F16_BASIS_AUTO_MIN_ROWS && n <= F16_BASIS_MAX_ROWS;
 }
 
 // xs[row*B + b] = (complex64) v[in_off[b] + row*B + b] for active slots
@@ -1145,6 +1153,7 @@ static SolveSpec solve_spec(const scadyn_operator *op) {
         spec.prec = SCADYN_PREC_MIXED;
         spec.verify64 = true;
         spec.tol_true = op->opts.refine_tol;
+        if (f16_basis(op)) spec.tol = std::max(spec.tol, F16_CYCLE_TOL);
     }
     return spec;
 }
