@@ -1,7 +1,7 @@
 """The Fortran host path: scadyn_amd_iface.f90 (ISO_C_BINDING) driving one
 wavelength bar the way the reference's calc_T does after the overlay.
 
-The driver executable is built by scadyn_amd/build.py with amdflang; the tests
+The Fortran host executable (solve_bar) is built by scadyn_amd/build.py with amdflang; the tests
 skip when it is absent.
 """
 import os
