@@ -12,8 +12,9 @@ accuracy:
     matvec, tol 1e-8 on |g(i+1)|/||b|| and up to 60 restart cycles, so that
     every column converges (the reference's default of 4 cycles stops all of
     them at the 200-iteration cap on this workload);
-  * mixed: FP32 matvec and Krylov basis (FP64 arithmetic) inside FP64
-    iterative refinement, done when the FP64 true residual is <= 1e-8 ||b||.
+  * mixed: FP32 matvec, Krylov basis stored in binary16 (FP64 arithmetic;
+    --basis), inside FP64 iterative refinement, done when the FP64 true
+    residual is <= 1e-8 ||b||.
 The relative Frobenius difference of the two legs' T columns is reported.
 
 A step is K converged columns (default K = 4): the batch keeps every slot busy
