@@ -108,8 +108,15 @@ template <>
 struct Prec<ch> {
     typedef float real;  // arithmetic on a stored element runs in float or double
     // double -> float -> binary16, both rounding to nearest even: the same
-    // result on the host and the device
-    __host__ __device__ static ch from(cd a) { return {(_Float16)(float)a.x, (_Float16)(float)a.y}; }
+    // result on the host and the device.  Contraction is off here so that the
+    // two conversions are not merged into one double -> binary16 conversion,
+    // which gfx950 has no instruction for (it becomes some 35 integer
+    // instructions per value instead of two conversions).
+    __host__ __device__ static ch from(cd a) {
+#pragma clang fp contract(off)
+        const float fx = (float)a.x, fy = (float)a.y;
+        return {(_Float16)fx, (_Float16)fy};
+    }
     __host__ __device__ static ch from(cf a) { return {(_Float16)a.x, (_Float16)a.y}; }
     __host__ __device__ static ch zero() { return {(_Float16)0.0f, (_Float16)0.0f}; }
 };

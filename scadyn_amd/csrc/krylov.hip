@@ -592,11 +592,19 @@ __global__ __launch_bounds__(256) void k_vec_update(KrylovDims d, const SlotStat
     const long long r0 = (long long)blockIdx.x * m.RW + m.rl, rs = (long long)gridDim.x * m.RW;
     if (s.act == ACT_V0 || s.act == ACT_NEXT) {
         const double h = (s.act == ACT_V0) ? s.beta : s.hnorm;
+        // a complex64 or binary16 basis takes w times one reciprocal per slot
+        // (the quotient is rounded to the stored precision anyway, and two FP64
+        // divisions per element made this sweep instruction-bound); the
+        // complex128 basis divides, as the reference does
+        const double hinv = h != 0.0 ? 1.0 / h : 1.0;
         VT *vt = V + (size_t)s.tgt * nB;
         for (long long row = r0; row < d.n; row += rs) {
             const size_t e = (size_t)row * d.B + b;
             cd w = Prec<cd>::from(W[e]);
-            if (h != 0.0) w = cmk(w.x / h, w.y / h);
+            if constexpr (sizeof(VT) < sizeof(cd))
+                w = cmk(w.x * hinv, w.y * hinv);
+            else if (h != 0.0)
+                w = cmk(w.x / h, w.y / h);
             vt[e] = Prec<VT>::from(w);
         }
     } else if (s.act == ACT_XUPD) {
