@@ -240,10 +240,11 @@ constexpr int MGS_BLOCK_AUTO = 4;
 // The mixed-precision solve keeps its Krylov basis in complex64 (arithmetic
 // stays FP64) when the blocked MGS runs: the basis is what its sweeps and the
 // complex64 matvec read.
-static bool f32_basis(const scadyn_operator *op) {
-    const int bs = op->opts.mgs_block == 0 ? MGS_BLOCK_AUTO : op->opts.mgs_block;
-    return op->opts.precision == SCADYN_PREC_MIXED && op->opts.ortho == 0 && bs > 1;
+static bool f32_basis_for(const scadyn_opts &o) {
+    const int bs = o.mgs_block == 0 ? MGS_BLOCK_AUTO : o.mgs_block;
+    return o.precision == SCADYN_PREC_MIXED && o.ortho == 0 && bs > 1;
 }
+static bool f32_basis(const scadyn_operator *op) { return f32_basis_for(op->opts); }
 // Of those, the ones whose basis is stored in binary16.  A unit basis vector
 // of n rows rounds with a relative error below 2^-11 per element; elements
 // under binary16's normal range (2^-14) add at most sqrt(n) 2^-25 to the
@@ -256,11 +257,21 @@ static bool f32_basis(const scadyn_operator *op) {
 constexpr long long F16_BASIS_MAX_ROWS = 1LL << 26;
 constexpr long long F16_BASIS_AUTO_MIN_ROWS = 1LL << 20;
 constexpr double F16_CYCLE_TOL = 1.0 / 1024.0;
-static bool f16_basis(const scadyn_operator *op) {
-    if (!f32_basis(op)) return false;
-    if (op->opts.basis == SCADYN_BASIS_F16) return true;
-    const long long n = 3LL * op->g.N;
-    return op->opts.basis == SCADYN_BASIS_AUTO && n >= F16_BASIS_AUTO_MIN_ROWS && n <= F16_BASIS_MAX_ROWS;
+static bool f16_basis_for(const scadyn_opts &o, long long n) {
+    if (!f32_basis_for(o)) return false;
+    if (o.basis == SCADYN_BASIS_F16) return true;
+    return o.basis == SCADYN_BASIS_AUTO && n >= F16_BASIS_AUTO_MIN_ROWS && n <= F16_BASIS_MAX_ROWS;
+}
+static bool f16_basis(const scadyn_operator *op) { return f16_basis_for(op->opts, 3LL * op->g.N); }
+
+// Krylov vectors of one slot of n rows with restart length m: the binary16
+// basis Vh with x, W, Rhs (complex128), the complex64 w copy and matvec input;
+// the complex64 basis Vf with x, W, Rhs and the w copy; or the complex128
+// basis with x, W, Rhs and one vector of slack
+static size_t slot_vec_bytes(const scadyn_opts &o, size_t n, int m) {
+    if (f16_basis_for(o, (long long)n)) return (size_t)(m + 1) * n * sizeof(ch) + n * (3 * sizeof(cd) + 2 * sizeof(cf));
+    if (f32_basis_for(o)) return (size_t)(m + 1) * n * sizeof(cf) + n * (3 * sizeof(cd) + sizeof(cf));
+    return (size_t)(m + 4) * n * sizeof(cd);
 }
 
 // xs[row*B + b] = (complex64) v[in_off[b] + row*B + b] for active slots
@@ -471,13 +482,7 @@ static void create_operator(scadyn_operator *op, const scadyn_mesh &mesh, const 
 static size_t slot_bytes(const scadyn_operator *op, int m) {
     const size_t n = 3 * (size_t)op->g.N;
     const size_t grid = (size_t)op->L[0] * op->L[1] * op->g.Nz * 4;
-    // complex64 basis Vf and w copy with complex128 x, W, Rhs; the binary16 basis
-    // Vh with the same and the complex64 matvec input Xs; or the complex128
-    // basis with x, W, Rhs and one vector of slack
-    const size_t vec = f16_basis(op)   ? (size_t)(m + 1) * n * sizeof(ch) + n * (3 * sizeof(cd) + 2 * sizeof(cf))
-                       : f32_basis(op) ? (size_t)(m + 1) * n * sizeof(cf) + n * (3 * sizeof(cd) + sizeof(cf))
-                                       : (size_t)(m + 4) * n * sizeof(cd);
-    return vec + 2 * grid * sizeof(cd);  // two grids (complex128 size: both precisions)
+    return slot_vec_bytes(op->opts, n, m) + 2 * grid * sizeof(cd);  // two grids (complex128 size: both precisions)
 }
 
 static void make_matvec_space(scadyn_operator *op, int B, MatvecSpace &ms) {
@@ -1350,14 +1355,13 @@ static Footprint footprint_of(const HostGeometry &g, long long P, const scadyn_o
     if (o.precision == SCADYN_PREC_MIXED) f.op += 864 * N + 8 * H[0] * H[1] * H[2];
     f.setup = 2 * cdb * L[0] * L[1] * L[2] + 9 * (size_t)P + cdb * 27 * 27 * 27 + cdb * 98 * 27;
     // solve workspace (workspace(), restart_space()): per slot the Krylov
-    // vectors (a complex64 basis for the mixed-precision blocked MGS) and two
+    // vectors (slot_vec_bytes: the basis storage of the solve) and two
     // complex128-sized grids; the reduction partials of up to 2048 row-sweep
     // workgroups; the mixed solve's narrow FP64 restart batch
     const int m = o.maxit;
     const bool mixed = o.precision == SCADYN_PREC_MIXED;
-    const bool vf = mixed && o.ortho == 0 && o.mgs_block != 1;
     const size_t grid = L[0] * L[1] * (size_t)g.Nz * 4;
-    const size_t vec = vf ? (size_t)(m + 1) * n * sizeof(cf) + n * (3 * cdb + sizeof(cf)) : (size_t)(m + 4) * n * cdb;
+    const size_t vec = slot_vec_bytes(o, n, m);
     size_t ws = (size_t)B * (vec + 2 * grid * cdb) + (size_t)B * 1024 * (m + 8) +
                 (size_t)2048 * (size_t)std::max(m + 1, 36) * B * cdb;
     if (mixed) {
@@ -1624,14 +1628,9 @@ void scadyn_opts_default(scadyn_opts *o) {
 
 const char *scadyn_last_error(void) { return g_err.c_str(); }
 
-int scadyn_op_create(const scadyn_mesh *mesh, const scadyn_opts *opts, double k, scadyn_operator **out) {
-    ABI_TRY
-    if (!mesh || !out) fail(SCADYN_ERR_ARG, "null argument");
-    scadyn_opts o;
-    if (opts)
-        o = *opts;
-    else
-        scadyn_opts_default(&o);
+// Solver settings an operator, a request or a footprint query accepts
+// (checked on the host, before anything reaches the device).
+static void check_opts(const scadyn_opts &o) {
     if (o.precision != SCADYN_PREC_FP64 && o.precision != SCADYN_PREC_MIXED)
         fail(SCADYN_ERR_ARG, "unknown precision mode");
     if (o.maxit < 1 || o.restart < 1) fail(SCADYN_ERR_ARG, "maxit and restart must be >= 1");
@@ -1651,6 +1650,17 @@ int scadyn_op_create(const scadyn_mesh *mesh, const scadyn_opts *opts, double k,
     if (o.mgs_block != 0 && o.mgs_block != 1 && o.mgs_block != 2 && o.mgs_block != 4 && o.mgs_block != 6 &&
         o.mgs_block != 8)
         fail(SCADYN_ERR_ARG, "mgs_block must be 0 (automatic), 1, 2, 4, 6 or 8");
+}
+
+int scadyn_op_create(const scadyn_mesh *mesh, const scadyn_opts *opts, double k, scadyn_operator **out) {
+    ABI_TRY
+    if (!mesh || !out) fail(SCADYN_ERR_ARG, "null argument");
+    scadyn_opts o;
+    if (opts)
+        o = *opts;
+    else
+        scadyn_opts_default(&o);
+    check_opts(o);
     *out = nullptr;
     std::unique_ptr<scadyn_operator> op(new scadyn_operator());
     create_operator(op.get(), *mesh, o, k);
@@ -2000,6 +2010,8 @@ int scadyn_footprint(const scadyn_mesh *mesh, const scadyn_opts *opts, int32_t n
         o = *opts;
     else
         scadyn_opts_default(&o);
+    if (nmax < 1) fail(SCADYN_ERR_ARG, "nmax must be >= 1");
+    check_opts(o);
     HostGeometry g;
     build_geometry(*mesh, o, g);
     NearPairs np;
@@ -2049,6 +2061,7 @@ int scadyn_server_submit(scadyn_server *sv, const scadyn_mesh *mesh, const scady
     if (r->opts.batch <= 0) r->opts.batch = std::min(ncol, 32);
     r->opts.device = sv->device;
     // validate and size on the host, before anything reaches the device
+    check_opts(r->opts);
     HostGeometry g;
     build_geometry(*mesh, r->opts, g);
     NearPairs np;

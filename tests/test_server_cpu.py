@@ -89,3 +89,37 @@ def test_host_only_cancel_queued_and_running():
     with pytest.raises(native.ScadynError, match='cancelled'):
         sv.wait(d)
     sv.close()
+
+
+def test_footprint_of_the_mixed_basis_storage():
+    """The footprint sizes the Krylov basis the solve will hold: binary16 when
+    asked for, complex64 under the automatic choice below 2^20 rows (this
+    mesh), and the complex128 basis of the FP64 solve whatever `basis` says."""
+    from scadyn_amd import native
+    c, e, p, k = sphere_case()
+    mix = dict(precision=native.PREC_MIXED, batch=16)
+    f_auto = native.footprint(c, e, p, 3, **mix)
+    f_c64 = native.footprint(c, e, p, 3, basis=native.BASIS_C64, **mix)
+    f_f16 = native.footprint(c, e, p, 3, basis=native.BASIS_F16, **mix)
+    assert f_auto == f_c64
+    # per slot: 51 basis vectors of 4 bytes less per element, one complex64
+    # vector more (the matvec input converted from the basis)
+    n = 3 * e.shape[0]
+    assert f_c64 - f_f16 == 16 * (51 * n * 4 - n * 8)
+    assert native.footprint(c, e, p, 3, basis=native.BASIS_F16) == native.footprint(c, e, p, 3)
+
+
+def test_invalid_options_fail_on_the_host():
+    """Option values the solver does not know are rejected by the footprint
+    query and at submission, before a request reaches the device."""
+    from scadyn_amd import native
+    c, e, p, k = sphere_case()
+    for bad in (dict(basis=7), dict(gather_kernel=2), dict(mgs_block=3), dict(precision=5)):
+        with pytest.raises(native.ScadynError) as ex:
+            native.footprint(c, e, p, 2, **bad)
+        assert ex.value.code == native.ERR_ARG
+    sv = native.Server(hbm_budget=1 << 40)
+    with pytest.raises(native.ScadynError) as ex:
+        sv.submit(c, e, p, k, 2, basis=7)
+    assert ex.value.code == native.ERR_ARG
+    sv.close()
