@@ -143,7 +143,7 @@ static void run_case(int B, int m, long long n, int nwg, std::mt19937_64 &rng) {
     }
     const bool ok = worst < 1e-12;
     if (!ok) ++failures;
-    std::printf("B=%3d m=%2d n=%5lld nwg=%3d passes=%d  worst rel.err %.2e %s\n", B, m, n, nwg, np + 1, worst,
+    std::printf("B=%3d m=%2d n=%5lld nwg=%4d passes=%d  worst rel.err %.2e %s\n", B, m, n, nwg, np + 1, worst,
                 ok ? "ok" : "FAIL");
 }
 
@@ -159,6 +159,10 @@ int main() {
             run_case(B, m, n, nwg, rng);
             run_case(B, m, n, 7, rng);  // more workgroups than rows per sweep for the wide batches
         }
+    // the launch geometry of the first large device run, which faulted (198k tets:
+    // B 112, m 50, 2048 row-sweep workgroups), over 3 rows more than one sweep
+    // step (2048 x 2 rows)
+    run_case(112, 50, 4099, 2048, rng);
     std::printf("%d failures\n", failures);
     return failures ? 1 : 0;
 }
