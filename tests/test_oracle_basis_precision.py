@@ -1,0 +1,82 @@
+"""The mixed solve's Krylov basis in binary16, restated on the oracle.
+
+The library's mixed solve (ColumnSolver with the FP64 true-residual check) is
+iterative refinement: each restart cycle runs GMRES(m) on the current FP64
+residual with the complex64 matvec, MGS in FP64 arithmetic, the vector being
+orthogonalised kept in complex64 and the basis stored at reduced precision;
+the FP64 true residual at every restart decides convergence.  Here that solver
+is written out in numpy around the oracle's matvec, with the basis rounded to
+complex64 or to binary16 when stored, on a 1,680-tet two-layer sphere at ka 3
+(an easy system whose cycles reach their target in a few dozen iterations).
+
+Checked:
+  * every variant reaches the FP64 target 1e-8 (accuracy is the true
+    residual's, whatever the stored precision);
+  * asked for a residual reduction of 1e-4 per cycle, the binary16 basis takes
+    more iterations than complex64: its rounding (2^-11 per element) limits a
+    cycle to about 2^-10, and a cycle asked for more runs on without gaining it;
+  * with the cycle target at 2^-10, as the library sets it for a binary16
+    basis, it takes at most 1.25x the iterations of complex64 at 1e-4.
+"""
+import numpy as np
+
+from oracle import pfft_ref
+from scadyn_amd import meshgen
+
+A_RADIUS = 1e-7
+F16_CYCLE_TOL = 2.0 ** -10
+
+
+def _store(v, prec):
+    if prec == 'c64':
+        return v.astype(np.complex64).astype(complex)
+    return v.real.astype(np.float16).astype(float) + 1j * v.imag.astype(np.float16).astype(float)
+
+
+def _refine(op, b, prec, cycle_tol, tol=1e-8, m=50, max_cycles=40):
+    """Matvecs to reach ||b - A x|| <= tol ||b|| (FP64), and the final ratio."""
+    A64 = lambda v: pfft_ref.matvec(op, v)
+    A32 = lambda v: pfft_ref.matvec(op, v.astype(np.complex64).astype(complex)).astype(np.complex64).astype(complex)
+    n = b.size
+    x = np.zeros(n, complex)
+    bn = np.linalg.norm(b)
+    its = 0
+    for _ in range(max_cycles):
+        r = b - A64(x)
+        beta = np.linalg.norm(r)
+        if beta <= tol * bn:
+            return its, beta / bn
+        V = np.zeros((n, m + 1), complex)
+        H = np.zeros((m + 1, m), complex)
+        V[:, 0] = _store(r / beta, prec)
+        for j in range(m):
+            w = A32(V[:, j])
+            its += 1
+            for i in range(j + 1):
+                H[i, j] = np.vdot(V[:, i], w)
+                w = (w - H[i, j] * V[:, i]).astype(np.complex64).astype(complex)
+            H[j + 1, j] = np.linalg.norm(w)
+            V[:, j + 1] = _store(w / H[j + 1, j], prec)
+            e1 = np.zeros(j + 2, complex)
+            e1[0] = beta
+            y = np.linalg.lstsq(H[:j + 2, :j + 1], e1, rcond=None)[0]
+            if np.linalg.norm(H[:j + 2, :j + 1] @ y - e1) < cycle_tol * beta:
+                break
+        x = x + V[:, :y.size] @ y
+    return its, np.linalg.norm(b - A64(x)) / bn
+
+
+def test_binary16_basis_with_its_cycle_target():
+    c, e = meshgen.sphere(8, seed=0)
+    c = meshgen.rescale_to_radius(c, e, A_RADIUS)
+    param = meshgen.layered_param(c, e, 0.6, 3.5 + 0.2j, 2.0 + 0.02j, a_scale=A_RADIUS)
+    k = 3.0 / A_RADIUS
+    op = pfft_ref.build_operator(c, e, param, k)
+    b = pfft_ref.vswf_matrix(c, e, k, 1)[:, 0]
+    it_c64, r_c64 = _refine(op, b, 'c64', 1e-4)
+    it_f16, r_f16 = _refine(op, b, 'f16', 1e-4)
+    it_f16c, r_f16c = _refine(op, b, 'f16', F16_CYCLE_TOL)
+    print(f'matvecs: complex64 {it_c64}, binary16 {it_f16}, binary16 with cycle target 2^-10 {it_f16c}')
+    assert max(r_c64, r_f16, r_f16c) <= 1e-8
+    assert it_f16 > it_c64
+    assert it_f16c <= 1.25 * it_c64
